@@ -1,0 +1,72 @@
+#!/bin/bash
+# Build switch variants of the reference treecode from its own sources, as
+# they lie under /root/reference, into oracle/_ref/ (binaries only).
+#
+# The reference selects behaviour with #define switches written into its
+# source (pot.c:25-217, pot-spheroid.c:25-230).  Each variant deletes the
+# shipped #define lines that must change and supplies the wanted values
+# with -D; the edited text is piped straight into the compiler, so no copy
+# of a reference source is written anywhere.  The build line is the
+# reference Makefile's (Makefile:9): mpicc -Os -std=c99 ... -lm, with gcc
+# as MPICH's back end.  The PRNG seed is read from ./seed.bin in the
+# directory the binary runs in (RANDOMNESS_SOURCE, pot.c:192).
+#
+# Usage: oracle/build_ref.sh [REFERENCE_DIR]
+set -euo pipefail
+
+REF=${1:-/root/reference}
+HERE="$(cd "$(dirname "$0")" && pwd)"
+OUT="$HERE/_ref"
+MPICC=${MPICC:-/opt/conda/bin/mpicc}
+
+if [ ! -f "$REF/pot.c" ]; then
+    echo "build_ref: no reference sources at $REF; skipping" >&2
+    exit 0
+fi
+if [ ! -x "$MPICC" ]; then
+    echo "build_ref: $MPICC not found; the reference needs MPI; skipping" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+
+# lines of the shipped pot.c that the variants replace
+POT_DEL=(-e '/^#define DISABLE_GRAIN$/d' -e '/^#define DEBUG_LANGMUIR 0.1$/d'
+         -e '/^#define OAP 0.8$/d' -e '/^#define EXTERNAL_B_[XYZ] 0.0$/d'
+         -e '/^#define RANDOMNESS_SOURCE/d')
+# lines of the shipped pot-spheroid.c that the variants replace (GL viewer
+# and its GLFW include removed, as the image has no GLFW)
+SPH_DEL=(-e '/^#define GLFW_INCLUDE_GLU/d' -e '/^#include <GLFW\/glfw.h>/d'
+         -e '/^#include <GL\/glu.h>/d' -e '/^#define GL$/d' -e '/^#define OAP 1.0$/d'
+         -e '/^#define EXTERNAL_B_[XYZ] 0.0$/d' -e '/^#define RANDOMNESS_SOURCE/d')
+
+build_pot() {  # name theta bx by bz [extra -D...]
+    local name=$1 theta=$2 bx=$3 by=$4 bz=$5
+    shift 5
+    sed "${POT_DEL[@]}" "$REF/pot.c" |
+        MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
+            -DSPHERICAL -DSCEPTIC_REINJECTION -DOAP="$theta" \
+            -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
+            -DRANDOMNESS_SOURCE='"seed.bin"' "$@" -o "$OUT/$name" -lm
+}
+
+build_sph() {  # name shape theta bx by bz
+    local name=$1 shape=$2 theta=$3 bx=$4 by=$5 bz=$6
+    local extra=()
+    if [ "$shape" = OBLATE ]; then extra=(-e '/^#define PROLATE$/d' ); fi
+    tr -d '\r' < "$REF/pot-spheroid.c" | sed "${SPH_DEL[@]}" "${extra[@]}" |
+        MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
+            -D"$shape" -DOAP="$theta" \
+            -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
+            -DRANDOMNESS_SOURCE='"seed.bin"' -o "$OUT/$name" -lm
+}
+
+# spherical grain, SCEPTIC reinjection, Boris
+build_pot pot_sphere 0.5 0.0 0.0 0.0
+build_pot pot_sphere_bz 0.5 0.0 0.0 50.0
+build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
+build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
+# spheroidal grain (pot-spheroid.c; config 3 uses B tilted 30 degrees off the x axis)
+build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
+build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0
+
+echo "build_ref: built $(ls "$OUT" | wc -l) reference variants in $OUT"

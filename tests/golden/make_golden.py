@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""Write the reference-run fixtures under tests/golden/.
+
+Runs the reference treecode, built from its own sources by
+oracle/build_ref.sh into oracle/_ref/, on small seeded cases and keeps what
+the reference itself writes: the format-C state file after 1 step and
+after K steps (pot.c:3816-3901) and pot-macroscopic.dat (pot.c:3247-3510).
+The PRNG seed is the 64-byte file ./seed.bin (RANDOMNESS_SOURCE, pot.c:192)
+expanded from an integer by splitmix64, so a test can rebuild it.
+
+Only this container has the reference; the fixtures travel, the reference
+does not.  Usage: python tests/golden/make_golden.py
+"""
+import json
+import math
+import os
+import shutil
+import struct
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_BIN = os.path.join(REPO, "oracle", "_ref")
+
+KB = 1.38065e-23
+EPS0 = 8.85419e-12
+QE = 1.6021766e-19
+ME = 9.109383e-31
+
+
+def splitmix64_words(seed, n=16):
+    """n little-endian u32 words from a splitmix64 stream (Steele et al. 2014)."""
+    out = []
+    x = seed & 0xFFFFFFFFFFFFFFFF
+    while len(out) < n:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        z ^= z >> 31
+        out.append(z & 0xFFFFFFFF)
+        out.append(z >> 32)
+    return out[:n]
+
+
+def seed_bytes(seed):
+    return struct.pack("<16I", *splitmix64_words(seed))
+
+
+def plasma_args(N, r_over_ld, te, ti, a_over_ld, soft_over_ld, dt_frac, r_px=1000):
+    """CLI values for a domain of R/lambda_D with N-1 plasma particles.
+
+    lambda_D follows the reference's convention n_e = (N-1)/(2V)
+    (pot.c:3658-3666); dt is dt_frac / omega_pe."""
+    n_e_count = (N - 1) // 2
+    R = 3.0 * n_e_count * QE * QE / (4 * math.pi * EPS0 * KB * te * r_over_ld ** 2)
+    ld = R / r_over_ld
+    wpe = math.sqrt(KB * te / ME) / ld
+    fmt = lambda v: "%.6g" % v
+    return {
+        "-N": str(N), "-r": str(r_px), "-m": fmt(R / r_px), "-a": fmt(a_over_ld * ld),
+        "-s": fmt(soft_over_ld * ld), "-d": fmt(dt_frac / wpe), "-E": fmt(te),
+        "-I": fmt(ti), "-e": "0",
+    }
+
+
+def cases():
+    base = plasma_args(601, 5.0, 11600.0, 11600.0, 0.5, 0.01, 0.01)
+    cs = []
+    cs.append({"name": "sphere", "bin": "pot_sphere", "geometry": 0, "theta": 0.5,
+               "B": [0.0, 0.0, 0.0], "args": dict(base), "steps": 100, "seed": 1})
+    cs.append({"name": "sphere_bz", "bin": "pot_sphere_bz", "geometry": 0, "theta": 0.5,
+               "B": [0.0, 0.0, 50.0], "args": dict(base), "steps": 100, "seed": 2})
+    drift = dict(base)
+    drift["-x"] = "%.6g" % math.sqrt(KB * 11600.0 / (1836.15 * ME))
+    cs.append({"name": "sphere_bx_drift", "bin": "pot_sphere_bx", "geometry": 0,
+               "theta": 0.5, "B": [30.0, 0.0, 0.0], "args": drift, "steps": 100, "seed": 3})
+    cs.append({"name": "sphere_t08", "bin": "pot_sphere_t08", "geometry": 0, "theta": 0.8,
+               "B": [0.0, 0.0, 0.0], "args": dict(base), "steps": 50, "seed": 4})
+    pro = plasma_args(601, 5.0, 11600.0, 1160.0, 0.8, 0.01, 0.01)
+    pro["-A"] = "2"
+    cs.append({"name": "prolate", "bin": "pot_prolate", "geometry": 1, "theta": 0.5,
+               "B": [43.30127018922193, 0.0, 25.0], "args": pro, "steps": 150, "seed": 5})
+    obl = plasma_args(601, 5.0, 11600.0, 11600.0, 0.5, 0.01, 0.01)
+    obl["-A"] = "0.5"
+    cs.append({"name": "oblate", "bin": "pot_oblate", "geometry": 2, "theta": 0.5,
+               "B": [0.0, 0.0, 0.0], "args": obl, "steps": 60, "seed": 6})
+    return cs
+
+
+def run_case(c, steps, workdir):
+    os.makedirs(os.path.join(workdir, "output"), exist_ok=True)
+    with open(os.path.join(workdir, "seed.bin"), "wb") as f:
+        f.write(seed_bytes(c["seed"]))
+    argv = [os.path.join(REF_BIN, c["bin"])]
+    for k, v in c["args"].items():
+        argv += [k, v]
+    argv += ["-i", str(steps), "-f", "state.bin"]
+    subprocess.run(argv, cwd=workdir, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, timeout=600)
+    return workdir
+
+
+def main():
+    if not os.path.isdir(REF_BIN):
+        sys.exit("build the reference variants first: oracle/build_ref.sh")
+    all_cases = cases()
+    for c in all_cases:
+        dest = os.path.join(HERE, c["name"])
+        os.makedirs(dest, exist_ok=True)
+        for steps, tag in ((1, "state_1.bin"), (c["steps"], "state_K.bin")):
+            wd = tempfile.mkdtemp(prefix="potgold_")
+            try:
+                run_case(c, steps, wd)
+                shutil.copy(os.path.join(wd, "state.bin"), os.path.join(dest, tag))
+                if tag == "state_K.bin":
+                    shutil.copy(os.path.join(wd, "output", "pot-macroscopic.dat"),
+                                os.path.join(dest, "macro.dat"))
+            finally:
+                shutil.rmtree(wd)
+        print("wrote", dest)
+    with open(os.path.join(HERE, "cases.json"), "w") as f:
+        json.dump(all_cases, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
