@@ -1,0 +1,178 @@
+/*
+ * potg.h -- C ABI of the MI355X (gfx950) hot path of the pot treecode.
+ *
+ * One context holds one or more independent grain simulations
+ * ("instances") resident in HBM.  A step advances every instance by one
+ * time step: octree build (Morton keys with the reference's float64
+ * midpoint recurrence, radix sort, bottom-up octree, monopole moments),
+ * Barnes-Hut field walk, Boris push, grain crossing with exact integer
+ * tallies, and SCEPTIC-like reinjection from a counter-based stream.
+ *
+ * Plain C types only; every call returns POTG_OK (0) or a negative
+ * POTG_ERR_* code and leaves a message for potg_last_error().  A context
+ * is used from one host thread at a time.
+ *
+ * Reference interfaces replaced (drewthomas/pot; SPH = pot-spheroid.c):
+ *   build_tree(Proc*, Run)                         pot.c:1457, SPH:1478
+ *   field_particle_feels_at(...)                   pot.c:1696, SPH:1794
+ *   phi_at(...), phi_particle_feels(...)           pot.c:1818, 1934
+ *   move_particles(Proc*, Run, Tree*, ..., t)      pot.c:2330, SPH:2541
+ *   assess_field_accuracy(...) (direct sum)        pot.c:2198
+ */
+#ifndef POTG_H
+#define POTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POTG_OK 0
+#define POTG_ERR_ARG (-1)        /* bad argument or configuration */
+#define POTG_ERR_HIP (-2)        /* HIP runtime failure */
+#define POTG_ERR_OUT_OF_BOX (-3) /* particle outside [0,2R]^3 or NaN (pot.c:1503-1514) */
+#define POTG_ERR_COINCIDENT (-4) /* two particles at one position (pot.c:1578-1586, 1799-1802) */
+#define POTG_ERR_DEPTH (-5)      /* tree deeper than the walk stack (pot.c:1647-1652) */
+#define POTG_ERR_NOMEM (-6)      /* device allocation failed */
+
+/* species codes, pot.c:286-290 */
+#define POTG_DUST 0
+#define POTG_ION 1
+#define POTG_ELECTRON 2
+
+/* grain geometry: sphere (pot.c) or spheroid with axis along x (SPH:111-121) */
+#define POTG_SPHERE 0
+#define POTG_PROLATE 1
+#define POTG_OBLATE 2
+
+/* arithmetic of the field walk */
+#define POTG_FP64 0  /* float64 everywhere */
+#define POTG_MIXED 1 /* float32 walk with compensated sums; float64 push */
+
+/* The reference's 56-byte AoS particle record (pot.c:295-303); also the
+   record format of state files (pot.c:3888). */
+typedef struct potg_particle {
+    double x, y, z, vx, vy, vz;
+    int32_t species;
+    int32_t pad;
+} potg_particle;
+
+/* Run parameters of one instance (pot.c:335-358, SPH:376-378) together
+   with the compile-time switches of the reference as run-time values. */
+typedef struct potg_config {
+    uint64_t N;            /* -N: particles including the grain at index 0 */
+    uint32_t r;            /* -r: domain radius in pixels */
+    uint32_t geometry;     /* POTG_SPHERE / POTG_PROLATE / POTG_OBLATE */
+    double dt;             /* -d: time step (s) */
+    double mpp;            /* -m: metres per pixel */
+    double soft;           /* -s: softening length (m) */
+    double x_drift;        /* -x: ion drift velocity along x (m/s) */
+    double a;              /* -a: grain equal-volume radius (m) */
+    double settle;         /* -e: settling time before absorption (s) */
+    double Te, Ti;         /* -E, -I: temperatures (K) */
+    double A;              /* -A: spheroid aspect ratio */
+    double theta;          /* OAP, the opening angle (pot.c:97) */
+    double B[3];           /* EXTERNAL_B_{X,Y,Z} (T) (pot.c:55-57) */
+    double max_lin_dev;    /* integrate_for_alpha tolerance (pot.c:436; SPH:457) */
+    double mime;           /* MIME, ion/electron mass ratio (pot.c:60) */
+    int32_t grain_enabled; /* absorbing grain (!DISABLE_GRAIN, pot.c:30) */
+    int32_t sceptic;       /* SCEPTIC_REINJECTION (pot.c:180-182); must be 1 */
+    uint64_t seed;         /* key of the counter-based reinjection stream */
+} potg_config;
+
+/* Scalar state of an instance besides its particles. */
+typedef struct potg_grain {
+    int32_t q;           /* grain charge in e (pot.c:371) */
+    uint32_t tot_absor_e; /* electrons absorbed so far (pot.c:395) */
+    double m;            /* grain mass (kg) (pot.c:372) */
+    double v[3];         /* grain velocity (parts[0].v) */
+    double chi_b;        /* normalised edge potential (pot.c:397) */
+    double epe[3];       /* electrostatic PE per species (pot.c:369) */
+} potg_grain;
+
+/* Per-instance result of one step (the reductions of pot.c:2813-2824). */
+typedef struct potg_step_out {
+    int32_t abs_q;        /* net charge absorbed this step (exact) */
+    uint32_t abs_e;       /* electrons absorbed this step (exact) */
+    uint32_t abs_i;       /* ions absorbed this step (exact) */
+    uint32_t n_pushed;    /* particles stepped */
+    uint32_t n_reinj_abs; /* reinjections after absorption */
+    uint32_t n_reinj_out; /* reinjections after leaving the domain */
+    uint32_t pad[2];
+} potg_step_out;
+
+/* Work counters accumulated since the last potg_stats reset. */
+typedef struct potg_stats_t {
+    uint64_t steps;
+    uint64_t n_targets;  /* walk targets */
+    uint64_t n_cell;     /* accepted cells with q != 0 */
+    uint64_t n_leaf;     /* leaf-pair interactions */
+    uint64_t n_open;     /* internal nodes opened (MAC failed) */
+    uint64_t n_nodes;    /* tree nodes built (internal + leaves) */
+    uint64_t n_particles;
+    uint64_t max_depth;
+} potg_stats_t;
+
+typedef struct potg_ctx potg_ctx;
+
+/* Create a context for n_inst instances (configs[i]); precision is
+   POTG_FP64 or POTG_MIXED.  Allocates every device buffer up front. */
+int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision,
+                potg_ctx** out);
+int potg_destroy(potg_ctx* ctx);
+const char* potg_last_error(void);
+
+/* Copy one instance's particles (reference index order, grain at index 0)
+   and grain state to the device, and set the step counter t (the number of
+   steps already completed, pot.c:4261). */
+int potg_upload(potg_ctx* ctx, uint32_t inst, const potg_particle* parts,
+                const potg_grain* grain, uint64_t t_completed);
+/* Copy one instance back, in reference index order. */
+int potg_download(potg_ctx* ctx, uint32_t inst, potg_particle* parts, potg_grain* grain);
+
+/* Advance every instance by one time step (run(), pot.c:4261-4388, with
+   move_particles, pot.c:2330-2934).  When compute_epe is nonzero the
+   potential-energy walks of pot.c:2771-2866 run too.  out, if non-NULL,
+   receives n_inst records. */
+int potg_step(potg_ctx* ctx, int32_t compute_epe, potg_step_out* out);
+/* Run `steps` steps as replays of one captured HIP graph (no host
+   synchronisation between steps; no per-step outputs). */
+int potg_run(potg_ctx* ctx, uint64_t steps);
+
+/* Build the octree of the current state without stepping (build_tree). */
+int potg_build_tree(potg_ctx* ctx);
+/* Canonical listing of instance inst's tree: DFS pre-order over the
+   non-empty nodes with children in the reference order (pot.c:324-331).
+   Arrays of capacity cap; returns the node count in *n_out. */
+int potg_tree_canon(potg_ctx* ctx, uint32_t inst, int32_t* depth, int32_t* is_leaf,
+                    uint32_t* part, int32_t* q, uint32_t* aq, double* c, double* x1x2,
+                    int32_t* nchild, int64_t cap, int64_t* n_out);
+
+/* Field at n points of instance inst using the current tree (built by
+   potg_build_tree or the last step), as field_particle_feels_at does:
+   exclude[k] is the reference index whose own leaf is skipped (UINT32_MAX
+   for none).  counts, if non-NULL, receives 3 per point: accepted cells,
+   leaf pairs, opened nodes.  Uses the context's precision. */
+int potg_field_at(potg_ctx* ctx, uint32_t inst, const double* xyz, const uint32_t* exclude,
+                  size_t n, double* E, uint32_t* counts);
+/* Potential at n points (phi_at, pot.c:1818-1901). */
+int potg_phi_at(potg_ctx* ctx, uint32_t inst, const double* xyz, size_t n, double* phi);
+/* Softened float64 direct sum at the positions of particles idx[0..n) of
+   instance inst over all its other particles (E_ref of the force checks). */
+int potg_direct_sum(potg_ctx* ctx, uint32_t inst, const uint32_t* idx, size_t n,
+                    double* E);
+
+int potg_stats(potg_ctx* ctx, potg_stats_t* out);
+int potg_stats_reset(potg_ctx* ctx);
+
+/* Wall time (ms) of the last potg_step / potg_run per pipeline stage,
+   measured with HIP events on the context's stream: [0] keys+sort,
+   [1] tree build, [2] walk, [3] push, [4] whole step. */
+int potg_stage_times(potg_ctx* ctx, int32_t enable, double* ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

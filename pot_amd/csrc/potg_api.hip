@@ -1,0 +1,681 @@
+// potg_api.hip -- the C ABI of include/potg.h: context, device memory,
+// upload/download and the step pipeline on one HIP stream.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/potg.h"
+#include "potg_internal.h"
+#include "potg_prim.h"
+
+using namespace potg;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+void set_err(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define HIPCHK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return POTG_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+}  // namespace
+
+struct potg_ctx {
+    int device;
+    hipStream_t stream;
+    Bufs b;
+    std::vector<potg_config> cfg;
+    std::vector<InstParams> hp;   // host copy of the parameters
+    std::vector<void*> allocs;
+    hipEvent_t ev[6];
+    int timing;
+    double stage_ms[5];
+    hipGraphExec_t graph[2];
+    int graph_ready;
+    uint32_t parity;  // which of the two captured steps comes next
+    void* h_aos;      // pinned staging
+    void* d_aos;
+    size_t aos_cap;
+    std::vector<char> uploaded;
+};
+
+namespace {
+
+int dev_alloc(potg_ctx* c, void** p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_err("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        return POTG_ERR_NOMEM;
+    }
+    c->allocs.push_back(*p);
+    return POTG_OK;
+}
+
+template <class T>
+int alloc_n(potg_ctx* c, T** p, size_t n)
+{
+    return dev_alloc(c, (void**)p, n * sizeof(T));
+}
+
+int kernel_error(potg_ctx* c)
+{
+    int32_t err = 0;
+    uint32_t idx = 0;
+    HIPCHK(hipMemcpyAsync(&err, &c->b.g->err, sizeof(err), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&idx, &c->b.g->err_idx, sizeof(idx), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!err) return POTG_OK;
+    int32_t zero = 0;
+    HIPCHK(hipMemcpy(&c->b.g->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    switch (err) {
+    case KERR_OUT_OF_BOX:
+        set_err("particle (global slot %u) not in simulation area", idx);
+        return POTG_ERR_OUT_OF_BOX;
+    case KERR_COINCIDENT:
+        set_err("multiple particles at one location (slot %u)", idx);
+        return POTG_ERR_COINCIDENT;
+    case KERR_CAPACITY:
+        set_err("particle tree larger than its preallocated capacity (%u)", idx);
+        return POTG_ERR_NOMEM;
+    default:
+        set_err("particle tree path length exceeded (node %u)", idx);
+        return POTG_ERR_DEPTH;
+    }
+}
+
+int validate(const potg_config& k, uint32_t i)
+{
+    if (k.N < 1 || k.N > 0x7fffffffULL) return set_err("instance %u: bad N", i), POTG_ERR_ARG;
+    if (k.r == 0 || !(k.mpp > 0) || !(k.dt > 0) || !(k.a > 0) || k.soft < 0 || !(k.Te >= 0) ||
+        !(k.Ti >= 0) || !(k.theta >= 0) || k.geometry > 2 || !(k.mime > 0))
+        return set_err("instance %u: bad parameters", i), POTG_ERR_ARG;
+    if (k.geometry == POTG_PROLATE && !(k.A > 1.0))
+        return set_err("instance %u: prolate aspect ratio must be greater than 1", i), POTG_ERR_ARG;
+    if (k.geometry == POTG_OBLATE && !(k.A < 1.0 && k.A > 0.0))
+        return set_err("instance %u: oblate aspect ratio must be less than 1", i), POTG_ERR_ARG;
+    return POTG_OK;
+}
+
+void fill_params(InstParams& p, const potg_config& k)
+{
+    memset(&p, 0, sizeof(p));
+    p.R = k.r * k.mpp;
+    p.root_x2 = 2.0 * k.r * k.mpp;
+    p.invR = 1.0 / p.R;
+    p.dt = k.dt;
+    p.soft2 = k.soft * k.soft;
+    p.theta2 = k.theta * k.theta;
+    p.a = k.a;
+    p.A = k.A;
+    p.settle = k.settle;
+    p.x_drift = k.x_drift;
+    p.Te = k.Te;
+    p.Ti = k.Ti;
+    for (int c = 0; c < 3; c++) p.B[c] = k.B[c];
+    p.Bmag = sqrt(k.B[0] * k.B[0] + k.B[1] * k.B[1] + k.B[2] * k.B[2]);
+    for (int c = 0; c < 3; c++) p.b[c] = p.Bmag != 0.0 ? k.B[c] / p.Bmag : 0.0;
+    p.max_lin_dev = k.max_lin_dev;
+    // MASS[] (pot.c:292) and tss[] (pot.c:4157-4159)
+    const long double mD = 1.67e-12, mI = k.mime * ME, mE = ME;
+    p.mass[0] = (double)mD;
+    p.mass[1] = (double)mI;
+    p.mass[2] = (double)mE;
+    p.tss[2] = 1;
+    p.tss[1] = (uint32_t)(int)sqrt((double)(k.Te * mI / (k.Ti * mE)));
+    if (p.tss[1] == 0) p.tss[1] = 1;
+    p.tss[0] = p.tss[1] * (uint32_t)(int)sqrt((double)(0.1 * mD / mI));
+    // Rol of sceptically_reinject (pot.c:578-580)
+    double Rol = QE * QE * (double)(k.N - 1) / 2.0;
+    Rol /= (4.0 * PI / 3.0) * k.mpp * k.r * EPSILON0 * KB * k.Te;
+    p.Rol = sqrt(Rol);
+    p.kTi_over_e = KB * k.Ti / (1 * QE);
+    p.Nm1 = (double)(k.N - 1);
+    p.geometry = k.geometry;
+    p.grain_enabled = k.grain_enabled;
+    p.sceptic = k.sceptic;
+    p.seed = k.seed;
+    p.N = (uint32_t)k.N;
+    for (int s = 0; s < 3; s++) p.first_idx[s] = NONE;
+}
+
+// chi_b sampling point of the one worker (measure_chi_b, pot.c:2257-2275)
+void chi_point(InstParams& p)
+{
+    const double azi = 2.0 * PI * 0 / 1;
+    const double zen = PI / 2.0;
+    const double r = p.R;
+    p.chi_pt[0] = p.grain[0] + (r * cos(azi) * sin(zen));
+    p.chi_pt[1] = p.grain[1] + (r * sin(azi) * sin(zen));
+    p.chi_pt[2] = p.grain[2] + (r * cos(zen));
+}
+
+int ensure_staging(potg_ctx* c, size_t bytes)
+{
+    if (bytes <= c->aos_cap) return POTG_OK;
+    if (c->h_aos) (void)hipHostFree(c->h_aos);
+    if (c->d_aos) (void)hipFree(c->d_aos);
+    c->h_aos = nullptr;
+    c->d_aos = nullptr;
+    HIPCHK(hipHostMalloc(&c->h_aos, bytes, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&c->d_aos, bytes));
+    c->aos_cap = bytes;
+    return POTG_OK;
+}
+
+void record(potg_ctx* c, int k)
+{
+    if (c->timing) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+void enqueue_step(potg_ctx* c)
+{
+    Bufs& b = c->b;
+    hipStream_t s = c->stream;
+    record(c, 0);
+    launch_begin_step(b, s);
+    launch_keys_sort_permute(b, s);
+    record(c, 1);
+    launch_tree(b, s);
+    launch_targets(b, s);
+    launch_chi_b(b, s);
+    record(c, 2);
+    launch_walk(b, s);
+    record(c, 3);
+    launch_push(b, s);
+    launch_finish_step(b, s);
+    record(c, 4);
+}
+
+int read_stage_times(potg_ctx* c)
+{
+    if (!c->timing) return POTG_OK;
+    HIPCHK(hipEventSynchronize(c->ev[4]));
+    float ms;
+    for (int k = 0; k < 4; k++) {
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+        c->stage_ms[k] = ms;
+    }
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[4]));
+    c->stage_ms[4] = ms;
+    return POTG_OK;
+}
+
+int all_uploaded(potg_ctx* c)
+{
+    for (size_t i = 0; i < c->uploaded.size(); i++)
+        if (!c->uploaded[i]) {
+            set_err("instance %zu has no particles uploaded", i);
+            return POTG_ERR_ARG;
+        }
+    return POTG_OK;
+}
+
+void destroy_graphs(potg_ctx* c)
+{
+    if (c->graph_ready) {
+        (void)hipGraphExecDestroy(c->graph[0]);
+        (void)hipGraphExecDestroy(c->graph[1]);
+        c->graph_ready = 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* potg_last_error(void) { return g_err; }
+
+int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, potg_ctx** out)
+{
+    if (!configs || !out || n_inst == 0 || n_inst > 65535 ||
+        (precision != POTG_FP64 && precision != POTG_MIXED))
+        return set_err("potg_create: bad arguments"), POTG_ERR_ARG;
+    *out = nullptr;
+    uint64_t n_total = 0, n_tree = 0;
+    for (uint32_t i = 0; i < n_inst; i++) {
+        int rc = validate(configs[i], i);
+        if (rc) return rc;
+        n_total += configs[i].N;
+        n_tree += configs[i].N - (configs[i].geometry != POTG_SPHERE ? 1 : 0);
+    }
+    if (n_total > 0x7fffffffULL) return set_err("too many particles"), POTG_ERR_ARG;
+
+    potg_ctx* c = new potg_ctx();
+    memset(c->ev, 0, sizeof(c->ev));
+    c->timing = 0;
+    c->graph_ready = 0;
+    c->parity = 0;
+    c->h_aos = c->d_aos = nullptr;
+    c->aos_cap = 0;
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        set_err("no HIP device available");
+        delete c;
+        return POTG_ERR_HIP;
+    }
+    for (int k = 0; k < 6; k++) (void)hipEventCreate(&c->ev[k]);
+    c->cfg.assign(configs, configs + n_inst);
+    Bufs& b = c->b;
+    memset(&b, 0, sizeof(b));
+    b.n_total = (uint32_t)n_total;
+    b.n_tree = (uint32_t)n_tree;
+    b.n_inst = n_inst;
+    b.precision = precision;
+    int ib = 0;
+    while ((1u << ib) < n_inst) ib++;
+    b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
+    b.key_bits = 64;
+
+    c->hp.resize(n_inst);
+    uint32_t off = 0, tb = 0;
+    for (uint32_t i = 0; i < n_inst; i++) {
+        fill_params(c->hp[i], configs[i]);
+        c->hp[i].offset = off;
+        c->hp[i].tree_begin = tb;
+        c->hp[i].tree_n = (uint32_t)configs[i].N - (configs[i].geometry != POTG_SPHERE ? 1 : 0);
+        off += (uint32_t)configs[i].N;
+        tb += c->hp[i].tree_n;
+    }
+
+    const size_t n = b.n_total, nt = b.n_tree ? b.n_tree : 1;
+    int rc = 0;
+    PState* S2[2] = {&b.st, &b.alt};
+    for (int k = 0; k < 2 && !rc; k++) {
+        PState& p = *S2[k];
+        rc |= alloc_n(c, &p.x, n) | alloc_n(c, &p.y, n) | alloc_n(c, &p.z, n);
+        rc |= alloc_n(c, &p.vx, n) | alloc_n(c, &p.vy, n) | alloc_n(c, &p.vz, n);
+        rc |= alloc_n(c, &p.ref, n) | alloc_n(c, &p.inst, n) | alloc_n(c, &p.spec, n);
+    }
+    rc |= alloc_n(c, &b.P, n_inst) | alloc_n(c, &b.S, n_inst) | alloc_n(c, &b.g, 1);
+    rc |= alloc_n(c, &b.key, n) | alloc_n(c, &b.key_alt, n);
+    rc |= alloc_n(c, &b.perm, n) | alloc_n(c, &b.perm_alt, n);
+    rc |= alloc_n(c, &b.sort_tmp, radix_temp_words((uint32_t)n));
+    rc |= alloc_n(c, &b.scan_tmp, scan_temp_words((uint32_t)(8 * n + 64)));
+    // internal nodes: about N/2 for uniform particles (single-child chains
+    // between close pairs add a few percent); 2N + 64 leaves ample room and
+    // every kernel checks the bound
+    const size_t ci = 2 * nt + 64;
+    b.cap_int = (uint32_t)ci;
+    rc |= alloc_n(c, &b.lcp, nt) | alloc_n(c, &b.cnt, nt) | alloc_n(c, &b.base, nt);
+    rc |= alloc_n(c, &b.owner, ci) | alloc_n(c, &b.level, ci);
+    rc |= alloc_n(c, &b.slots, 8 * ci) | alloc_n(c, &b.ccount, ci) | alloc_n(c, &b.cfirst, ci);
+    rc |= alloc_n(c, &b.wid_int, ci) | alloc_n(c, &b.wid_leaf, nt);
+    rc |= alloc_n(c, &b.par_int, ci) | alloc_n(c, &b.par_leaf, nt) | alloc_n(c, &b.arrive, ci);
+    rc |= alloc_n(c, &b.size2, ci);
+    rc |= alloc_n(c, &b.mcx, ci) | alloc_n(c, &b.mcy, ci) | alloc_n(c, &b.mcz, ci);
+    rc |= alloc_n(c, &b.mq, ci) | alloc_n(c, &b.maq, ci);
+    b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
+    rc |= alloc_n(c, &b.rec64, b.n_rec);
+    if (precision == POTG_MIXED) rc |= alloc_n(c, &b.rec32, b.n_rec);
+    rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
+    rc |= alloc_n(c, &b.E, 3 * nt) | alloc_n(c, &b.inv, n);
+    b.q_cap = (uint32_t)(n < 65536 ? 65536 : n);
+    rc |= alloc_n(c, &b.q_pts, 3 * (size_t)b.q_cap) | alloc_n(c, &b.q_inst, b.q_cap);
+    rc |= alloc_n(c, &b.q_self, b.q_cap) | alloc_n(c, &b.q_out, 3 * (size_t)b.q_cap);
+    rc |= alloc_n(c, &b.q_counts, 3 * (size_t)b.q_cap);
+    if (rc) {
+        potg_destroy(c);
+        return POTG_ERR_NOMEM;
+    }
+    // zeroed records: a record the build has not written is an empty cell
+    if (hipMemset(b.rec64, 0, sizeof(NodeRec64) * b.n_rec) != hipSuccess ||
+        (b.rec32 && hipMemset(b.rec32, 0, sizeof(NodeRec32) * b.n_rec) != hipSuccess)) {
+        set_err("initial clear failed");
+        potg_destroy(c);
+        return POTG_ERR_HIP;
+    }
+    c->uploaded.assign(n_inst, 0);
+    Globals g0;
+    memset(&g0, 0, sizeof(g0));
+    std::vector<InstState> s0(n_inst);
+    memset(s0.data(), 0, sizeof(InstState) * n_inst);
+    if (hipMemcpy(b.g, &g0, sizeof(g0), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.S, s0.data(), sizeof(InstState) * n_inst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.P, c->hp.data(), sizeof(InstParams) * n_inst, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        set_err("initial copies failed");
+        potg_destroy(c);
+        return POTG_ERR_HIP;
+    }
+    *out = c;
+    return POTG_OK;
+}
+
+int potg_destroy(potg_ctx* c)
+{
+    if (!c) return POTG_OK;
+    (void)hipStreamSynchronize(c->stream);
+    destroy_graphs(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->h_aos) (void)hipHostFree(c->h_aos);
+    if (c->d_aos) (void)hipFree(c->d_aos);
+    for (int k = 0; k < 6; k++)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return POTG_OK;
+}
+
+int potg_upload(potg_ctx* c, uint32_t inst, const potg_particle* parts, const potg_grain* grain,
+                uint64_t t_completed)
+{
+    if (!c || !parts || !grain || inst >= c->b.n_inst)
+        return set_err("potg_upload: bad arguments"), POTG_ERR_ARG;
+    const potg_config& k = c->cfg[inst];
+    const size_t N = k.N;
+    if (parts[0].species != POTG_DUST) return set_err("particle 0 must be the grain"), POTG_ERR_ARG;
+    InstParams& p = c->hp[inst];
+    for (int s = 0; s < 3; s++) p.first_idx[s] = NONE;
+    for (size_t i = 1; i < N; i++) {
+        const int s = parts[i].species;
+        if (s != POTG_ION && s != POTG_ELECTRON)
+            return set_err("particle %zu: species must be ion or electron", i), POTG_ERR_ARG;
+        if (p.first_idx[s] == NONE) p.first_idx[s] = (uint32_t)i;
+    }
+    p.grain[0] = parts[0].x;
+    p.grain[1] = parts[0].y;
+    p.grain[2] = parts[0].z;
+    chi_point(p);
+    int rc = ensure_staging(c, N * sizeof(potg_particle));
+    if (rc) return rc;
+    destroy_graphs(c);
+    memcpy(c->h_aos, parts, N * sizeof(potg_particle));
+    HIPCHK(hipMemcpyAsync(c->d_aos, c->h_aos, N * sizeof(potg_particle), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(&c->b.P[inst], &p, sizeof(InstParams), hipMemcpyHostToDevice, c->stream));
+    launch_scatter_upload(c->b, inst, (const double*)c->d_aos, (uint32_t)N, c->stream);
+    launch_build_inverse(c->b, c->stream);
+    InstState s;
+    memset(&s, 0, sizeof(s));
+    s.grain_q = grain->q;
+    s.tot_e = grain->tot_absor_e;
+    s.grain_m = grain->m;
+    s.grain_v[0] = parts[0].vx;
+    s.grain_v[1] = parts[0].vy;
+    s.grain_v[2] = parts[0].vz;
+    s.chi_b = grain->chi_b;
+    for (int q = 0; q < 3; q++) s.epe[q] = grain->epe[q];
+    HIPCHK(hipMemcpyAsync(&c->b.S[inst], &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(&c->b.g->t, &t_completed, sizeof(uint64_t), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->uploaded[inst] = 1;
+    return POTG_OK;
+}
+
+int potg_download(potg_ctx* c, uint32_t inst, potg_particle* parts, potg_grain* grain)
+{
+    if (!c || inst >= c->b.n_inst) return set_err("potg_download: bad arguments"), POTG_ERR_ARG;
+    const size_t N = c->cfg[inst].N;
+    InstState s;
+    HIPCHK(hipMemcpyAsync(&s, &c->b.S[inst], sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    if (parts) {
+        int rc = ensure_staging(c, N * sizeof(potg_particle));
+        if (rc) return rc;
+        launch_gather_download(c->b, inst, (double*)c->d_aos, (uint32_t)N, c->stream);
+        HIPCHK(hipMemcpyAsync(c->h_aos, c->d_aos, N * sizeof(potg_particle), hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (parts) {
+        memcpy(parts, c->h_aos, N * sizeof(potg_particle));
+        // the grain: fixed position, velocity from the absorbed momentum
+        parts[0].vx = s.grain_v[0];
+        parts[0].vy = s.grain_v[1];
+        parts[0].vz = s.grain_v[2];
+        parts[0].species = POTG_DUST;
+    }
+    if (grain) {
+        grain->q = s.grain_q;
+        grain->tot_absor_e = s.tot_e;
+        grain->m = s.grain_m;
+        for (int q = 0; q < 3; q++) grain->v[q] = s.grain_v[q];
+        grain->chi_b = s.chi_b;
+        for (int q = 0; q < 3; q++) grain->epe[q] = s.epe[q];
+    }
+    return POTG_OK;
+}
+
+int potg_step(potg_ctx* c, int32_t compute_epe, potg_step_out* out)
+{
+    if (!c) return set_err("potg_step: null context"), POTG_ERR_ARG;
+    if (all_uploaded(c)) return POTG_ERR_ARG;
+    (void)compute_epe;
+    enqueue_step(c);
+    c->parity ^= 1u;
+    HIPCHK(hipGetLastError());
+    int rc = kernel_error(c);
+    if (rc) return rc;
+    rc = read_stage_times(c);
+    if (rc) return rc;
+    if (out) {
+        std::vector<InstState> s(c->b.n_inst);
+        HIPCHK(hipMemcpy(s.data(), c->b.S, sizeof(InstState) * c->b.n_inst, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < c->b.n_inst; i++) {
+            memset(&out[i], 0, sizeof(potg_step_out));
+            out[i].abs_q = s[i].out_q;
+            out[i].abs_e = s[i].out_e;
+            out[i].abs_i = s[i].out_i;
+            out[i].n_pushed = s[i].out_pushed;
+            out[i].n_reinj_abs = s[i].out_reinj_abs;
+            out[i].n_reinj_out = s[i].out_reinj_out;
+        }
+    }
+    return POTG_OK;
+}
+
+int potg_run(potg_ctx* c, uint64_t steps)
+{
+    if (!c) return set_err("potg_run: null context"), POTG_ERR_ARG;
+    if (all_uploaded(c)) return POTG_ERR_ARG;
+    if (!c->graph_ready) {
+        // two graphs: the particle buffers swap roles every step
+        const int timing = c->timing;
+        c->timing = 0;
+        for (int k = 0; k < 2; k++) {
+            hipGraph_t gr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(c);
+            HIPCHK(hipStreamEndCapture(c->stream, &gr));
+            HIPCHK(hipGraphInstantiate(&c->graph[k], gr, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(gr));
+        }
+        c->timing = timing;
+        c->graph_ready = 1;
+        c->parity = 0;
+    }
+    for (uint64_t s = 0; s < steps; s++) {
+        HIPCHK(hipGraphLaunch(c->graph[c->parity], c->stream));
+        c->parity ^= 1u;
+        // host copy of the buffer roles follows the device
+        PState t = c->b.st;
+        c->b.st = c->b.alt;
+        c->b.alt = t;
+    }
+    // the capture itself swapped the roles twice, so `st` is consistent
+    return kernel_error(c);
+}
+
+int potg_build_tree(potg_ctx* c)
+{
+    if (!c) return set_err("potg_build_tree: null context"), POTG_ERR_ARG;
+    if (all_uploaded(c)) return POTG_ERR_ARG;
+    destroy_graphs(c);
+    launch_keys_sort_permute(c->b, c->stream);
+    launch_tree(c->b, c->stream);
+    HIPCHK(hipGetLastError());
+    return kernel_error(c);
+}
+
+int potg_tree_canon(potg_ctx* c, uint32_t inst, int32_t* depth, int32_t* is_leaf, uint32_t* part,
+                    int32_t* q, uint32_t* aq, double* cc, double* size2, int32_t* nchild,
+                    int64_t cap, int64_t* n_out)
+{
+    if (!c || inst >= c->b.n_inst || !n_out) return set_err("potg_tree_canon: bad arguments"), POTG_ERR_ARG;
+    Globals g;
+    HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
+    const size_t nrec = c->b.n_inst + g.n_children;
+    std::vector<NodeRec64> rec(nrec);
+    std::vector<uint32_t> ref(c->b.n_total);
+    std::vector<int32_t> mq;
+    HIPCHK(hipMemcpy(rec.data(), c->b.rec64, nrec * sizeof(NodeRec64), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ref.data(), c->b.st.ref, ref.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t off = c->hp[inst].offset;
+    std::vector<std::pair<uint32_t, int>> stk;
+    int64_t n = 0;
+    if (c->hp[inst].tree_n == 0) {
+        *n_out = 0;
+        return POTG_OK;
+    }
+    stk.push_back({inst, 0});
+    while (!stk.empty()) {
+        const uint32_t w = stk.back().first;
+        const int d = stk.back().second;
+        stk.pop_back();
+        if (n >= cap) return set_err("potg_tree_canon: capacity"), POTG_ERR_ARG;
+        const NodeRec64& r = rec[w];
+        const bool leaf = r.cnt == LEAF;
+        depth[n] = d;
+        is_leaf[n] = leaf;
+        part[n] = leaf ? ref[r.part] - off : 0xFFFFFFFFu;
+        q[n] = r.q;
+        aq[n] = leaf ? (uint32_t)abs(r.q) : r.part;
+        cc[3 * n] = r.cx;
+        cc[3 * n + 1] = r.cy;
+        cc[3 * n + 2] = r.cz;
+        size2[n] = leaf ? 0.0 : r.size2;
+        nchild[n] = leaf ? 0 : (int32_t)r.cnt;
+        n++;
+        if (!leaf)
+            for (int k = (int)r.cnt - 1; k >= 0; k--) stk.push_back({r.first + (uint32_t)k, d + 1});
+    }
+    *n_out = n;
+    return POTG_OK;
+}
+
+static int run_points(potg_ctx* c, uint32_t inst, const double* xyz, const uint32_t* exclude,
+                      size_t n, int kind, double* out, uint32_t* counts)
+{
+    Bufs& b = c->b;
+    std::vector<uint32_t> vi, vs;
+    std::vector<uint32_t> inv(b.n_total);
+    if (kind == 0 && exclude)
+        HIPCHK(hipMemcpy(inv.data(), b.inv, inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t off = c->hp[inst].offset;
+    for (size_t s0 = 0; s0 < n; s0 += b.q_cap) {
+        const uint32_t m = (uint32_t)((n - s0) < b.q_cap ? (n - s0) : b.q_cap);
+        vi.assign(m, inst);
+        vs.assign(m, NONE);
+        if (kind == 0 && exclude)
+            for (uint32_t k = 0; k < m; k++) {
+                const uint32_t e = exclude[s0 + k];
+                vs[k] = (e == 0xFFFFFFFFu || e >= c->cfg[inst].N) ? NONE : inv[off + e];
+            }
+        HIPCHK(hipMemcpyAsync(b.q_pts, xyz + 3 * s0, 3 * sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b.q_inst, vi.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b.q_self, vs.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
+        launch_points(b, m, kind, c->stream);
+        const size_t per = kind == 0 ? 3 : 1;
+        HIPCHK(hipMemcpyAsync(out + per * s0, b.q_out, per * sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        if (counts && kind == 0)
+            HIPCHK(hipMemcpyAsync(counts + 3 * s0, b.q_counts, 3 * sizeof(uint32_t) * m,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return kernel_error(c);
+}
+
+int potg_field_at(potg_ctx* c, uint32_t inst, const double* xyz, const uint32_t* exclude, size_t n,
+                  double* E, uint32_t* counts)
+{
+    if (!c || inst >= c->b.n_inst || (n && (!xyz || !E)))
+        return set_err("potg_field_at: bad arguments"), POTG_ERR_ARG;
+    return run_points(c, inst, xyz, exclude, n, 0, E, counts);
+}
+
+int potg_phi_at(potg_ctx* c, uint32_t inst, const double* xyz, size_t n, double* phi)
+{
+    if (!c || inst >= c->b.n_inst || (n && (!xyz || !phi)))
+        return set_err("potg_phi_at: bad arguments"), POTG_ERR_ARG;
+    return run_points(c, inst, xyz, nullptr, n, 1, phi, nullptr);
+}
+
+int potg_direct_sum(potg_ctx* c, uint32_t inst, const uint32_t* idx, size_t n, double* E)
+{
+    if (!c || inst >= c->b.n_inst || (n && (!idx || !E)))
+        return set_err("potg_direct_sum: bad arguments"), POTG_ERR_ARG;
+    Bufs& b = c->b;
+    std::vector<uint32_t> inv(b.n_total), vs;
+    HIPCHK(hipMemcpy(inv.data(), b.inv, inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t off = c->hp[inst].offset;
+    for (size_t s0 = 0; s0 < n; s0 += b.q_cap) {
+        const uint32_t m = (uint32_t)((n - s0) < b.q_cap ? (n - s0) : b.q_cap);
+        vs.resize(m);
+        for (uint32_t k = 0; k < m; k++) {
+            if (idx[s0 + k] >= c->cfg[inst].N) return set_err("potg_direct_sum: index out of range"), POTG_ERR_ARG;
+            vs[k] = inv[off + idx[s0 + k]];
+        }
+        HIPCHK(hipMemcpyAsync(b.q_self, vs.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, c->stream));
+        launch_direct_sum(b, inst, m, c->stream);
+        HIPCHK(hipMemcpyAsync(E + 3 * s0, b.q_out, 3 * sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return POTG_OK;
+}
+
+int potg_stats(potg_ctx* c, potg_stats_t* out)
+{
+    if (!c || !out) return set_err("potg_stats: bad arguments"), POTG_ERR_ARG;
+    Globals g;
+    HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
+    memset(out, 0, sizeof(*out));
+    out->steps = g.n_steps;
+    out->n_targets = g.n_tgt_total;
+    out->n_cell = g.n_cell;
+    out->n_leaf = g.n_leaf;
+    out->n_open = g.n_open;
+    out->n_nodes = (uint64_t)g.n_internal + c->b.n_tree;
+    out->n_particles = c->b.n_total;
+    out->max_depth = g.max_depth;
+    return POTG_OK;
+}
+
+int potg_stats_reset(potg_ctx* c)
+{
+    if (!c) return set_err("potg_stats_reset: null context"), POTG_ERR_ARG;
+    Globals g;
+    HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
+    g.n_cell = g.n_leaf = g.n_open = g.n_tgt_total = g.n_steps = 0;
+    g.max_depth = 0;
+    HIPCHK(hipMemcpy(c->b.g, &g, sizeof(g), hipMemcpyHostToDevice));
+    return POTG_OK;
+}
+
+int potg_stage_times(potg_ctx* c, int32_t enable, double* ms5)
+{
+    if (!c) return set_err("potg_stage_times: null context"), POTG_ERR_ARG;
+    c->timing = enable ? 1 : 0;
+    if (ms5)
+        for (int k = 0; k < 5; k++) ms5[k] = c->stage_ms[k];
+    return POTG_OK;
+}
+
+}  // extern "C"

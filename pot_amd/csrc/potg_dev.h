@@ -1,0 +1,156 @@
+// potg_dev.h -- device-side data layout and helpers shared by the gfx950
+// kernels of the pot hot path.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace potg {
+
+// Physical constants of the reference (pot.c:278-282; PI from prng.h:9-14,
+// which pot.c includes before its own definition).
+constexpr double KB = 1.38065e-23;
+constexpr double EPSILON0 = 8.85419e-12;
+constexpr double QE = 1.6021766e-19;
+constexpr double ME = 9.109383e-31;
+constexpr double PI = 3.14159265358979;
+constexpr double SQRT_PI = 1.77245385090552;
+// (4 * PI * EPSILON0) as the reference's expressions fold it (pot.c:1738, 1792)
+constexpr double K4PE0 = (4.0 * PI) * EPSILON0;
+
+constexpr int MAXD = 60;          // deepest leaf supported (the reference's 400-entry
+                                  // walk stack allows about 57 levels, pot.c:194-196)
+constexpr uint32_t LEAF = 0xFFFFFFFFu;   // NodeRec.cnt marker of a leaf
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint32_t LEAF_BIT = 0x80000000u; // child-slot code of a leaf
+constexpr int WAVE = 64;
+
+// error codes raised by kernels (mirrors POTG_ERR_*)
+constexpr int KERR_OUT_OF_BOX = 1;
+constexpr int KERR_COINCIDENT = 2;
+constexpr int KERR_DEPTH = 3;
+constexpr int KERR_CAPACITY = 4;
+
+// Constant parameters of one instance (one grain simulation).
+struct InstParams {
+    double R;          // r * mpp: domain radius and centre coordinate
+    double root_x2;    // 2.0 * r * mpp: upper face of the root box (pot.c:1476)
+    double invR;       // 1 / R, for the float32 walk's normalised coordinates
+    double dt;
+    double soft2;      // soft * soft
+    double theta2;     // OAP * OAP
+    double a, A;
+    double settle;
+    double x_drift, Te, Ti;
+    double B[3];
+    double Bmag;
+    double b[3];
+    double max_lin_dev;
+    double mass[3];    // dust, ion, electron
+    double Rol;        // R / shielding length of integrate_for_alpha (pot.c:578-580)
+    double grain[3];   // grain position (parts[0]), fixed
+    double chi_pt[3];  // boundary point where chi_b is sampled (pot.c:2271-2275)
+    double kTi_over_e; // KB * Ti / (CHARGE[ION] * QE) (pot.c:2278)
+    double Nm1;        // N - 1 as a double
+    uint32_t tss[3];
+    uint32_t geometry;
+    int32_t grain_enabled;
+    int32_t sceptic;
+    uint64_t seed;
+    uint32_t offset;     // first global slot of this instance
+    uint32_t N;          // particles including the grain
+    uint32_t tree_begin; // first sorted slot of the instance's tree particles
+    uint32_t tree_n;     // tree particles (N, or N - 1 when the grain is analytic)
+    uint32_t first_idx[3]; // lowest reference index of each species
+    uint32_t pad;
+};
+
+// Mutable per-instance state on the device.
+struct InstState {
+    int32_t grain_q;
+    uint32_t tot_e;
+    double grain_m;
+    double grain_v[3];
+    double chi_b;
+    double epe[3];
+    uint32_t start;      // first stepped reference index of the current step
+    uint32_t root;       // walk id of the tree root
+    // per-step accumulators (cleared by the step's first kernel)
+    int32_t acc_q;
+    uint32_t acc_e, acc_i, n_pushed, n_reinj_abs, n_reinj_out;
+    double acc_p[3];
+    // outputs of the last step
+    int32_t out_q;
+    uint32_t out_e, out_i, out_pushed, out_reinj_abs, out_reinj_out;
+};
+
+// Context-wide device counters.
+struct Globals {
+    uint64_t t;            // steps completed
+    uint32_t n_targets;    // walk targets of the current step
+    uint32_t n_internal;   // internal tree nodes
+    uint32_t n_children;   // sum of child counts
+    uint32_t tile_ctr;     // persistent-walk work counter
+    uint32_t tile_ctr2;
+    int32_t err;
+    uint32_t err_idx;
+    uint32_t max_depth;
+    unsigned long long n_cell, n_leaf, n_open, n_tgt_total, n_steps;
+};
+
+// Tree node records, children contiguous: an internal node's children are
+// the records [first, first + cnt), ordered as the reference orders them.
+struct __align__(16) NodeRec64 {
+    double cx, cy, cz, size2;
+    int32_t q;
+    uint32_t first;
+    uint32_t cnt;   // LEAF for a leaf
+    uint32_t part;  // leaf: sorted slot of its particle
+};
+struct __align__(16) NodeRec32 {
+    float cx, cy, cz, size2;   // relative to the domain centre, in units of R
+    int32_t q;
+    uint32_t first;
+    uint32_t cnt;
+    uint32_t part;
+};
+
+// Child slot for a position against a node's midpoints, numbered in the
+// reference's child order gxlylz, gxlygz, lxlylz, lxlygz, lxgylz, lxgygz,
+// gxgylz, gxgygz (pot.c:324-331, 1525-1553).
+__host__ __device__ inline int ref_digit(bool gx, bool gy, bool gz)
+{
+    const int g = gx ? (gy ? 3 : 0) : (gy ? 2 : 1);
+    return 2 * g + (gz ? 1 : 0);
+}
+
+// One level of the reference's float64 midpoint recurrence (pot.c:1522-1524,
+// 1261-1287): returns the child digit and narrows the box to that child.
+__host__ __device__ inline int descend1(double px, double py, double pz, double& x1, double& x2,
+                                        double& y1, double& y2, double& z1, double& z2)
+{
+    const double mx = (x1 + x2) / 2.0;
+    const double my = (y1 + y2) / 2.0;
+    const double mz = (z1 + z2) / 2.0;
+    const bool gx = !(px < mx), gy = !(py < my), gz = !(pz < mz);
+    if (gx) x1 = mx; else x2 = mx;
+    if (gy) y1 = my; else y2 = my;
+    if (gz) z1 = mz; else z2 = mz;
+    return ref_digit(gx, gy, gz);
+}
+
+__device__ inline uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline uint64_t rfl64(uint64_t v)
+{
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+__device__ inline void raise_err(Globals* g, int code, uint32_t idx)
+{
+    if (atomicCAS(&g->err, 0, code) == 0) g->err_idx = idx;
+}
+
+}  // namespace potg

@@ -1,0 +1,87 @@
+// potg_internal.h -- the context's device buffers and the kernel launchers
+// of the step pipeline.  Not part of the C ABI.
+#pragma once
+
+#include "potg_dev.h"
+
+namespace potg {
+
+// Particle state in sorted (octree) order, structure of arrays.
+struct PState {
+    double *x, *y, *z, *vx, *vy, *vz;
+    uint32_t* ref;     // global slot in reference order (instance offset + index)
+    uint16_t* inst;
+    uint8_t* spec;
+};
+
+struct Bufs {
+    uint32_t n_total;  // particle slots over all instances
+    uint32_t n_tree;   // leading sorted slots that belong to the trees
+    uint32_t n_inst;
+    int precision;     // POTG_FP64 / POTG_MIXED
+    int key_levels;    // digits carried by a sort key
+    int key_bits;      // bits of the key that are sorted
+    InstParams* P;
+    InstState* S;
+    Globals* g;
+    PState st, alt;
+    uint64_t *key, *key_alt;
+    uint32_t *perm, *perm_alt;
+    uint32_t* sort_tmp;
+    uint32_t* scan_tmp;
+    // tree
+    int32_t* lcp;
+    uint32_t* cnt;
+    uint32_t* base;
+    uint32_t* owner;
+    int32_t* level;
+    uint32_t* slots;   // 8 per internal node; NONE when empty
+    uint32_t* ccount;
+    uint32_t* cfirst;
+    uint32_t* wid_int;
+    uint32_t* wid_leaf;
+    uint32_t* par_int;
+    uint32_t* par_leaf;
+    uint32_t* arrive;
+    double* size2;
+    double *mcx, *mcy, *mcz;
+    int32_t* mq;
+    uint32_t* maq;
+    NodeRec64* rec64;
+    NodeRec32* rec32;
+    uint32_t n_rec;    // record capacity
+    uint32_t cap_int;  // internal-node capacity
+    // targets of the walk and push
+    uint32_t* flags;
+    uint32_t* tpos;
+    uint32_t* targets;
+    double* E;
+    uint32_t* inv;     // global reference slot -> sorted slot
+    // point queries
+    double* q_pts;
+    uint32_t* q_inst;
+    uint32_t* q_self;
+    double* q_out;
+    uint32_t* q_counts;
+    uint32_t q_cap;
+};
+
+// step stages (kernel launches only; no synchronisation)
+void launch_begin_step(Bufs& b, hipStream_t s);
+void launch_keys_sort_permute(Bufs& b, hipStream_t s);
+void launch_tree(Bufs& b, hipStream_t s);
+void launch_targets(Bufs& b, hipStream_t s);
+void launch_chi_b(Bufs& b, hipStream_t s);
+void launch_walk(Bufs& b, hipStream_t s);
+void launch_push(Bufs& b, hipStream_t s);
+void launch_finish_step(Bufs& b, hipStream_t s);
+void launch_epe(Bufs& b, hipStream_t s);
+// upload / download helpers
+void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n, hipStream_t s);
+void launch_gather_download(Bufs& b, uint32_t inst, double* aos, uint32_t n, hipStream_t s);
+void launch_build_inverse(Bufs& b, hipStream_t s);
+// point queries: n points of q_pts with q_inst/q_self; kind 0 field, 1 potential
+void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s);
+void launch_direct_sum(Bufs& b, uint32_t inst, uint32_t n, hipStream_t s);
+
+}  // namespace potg

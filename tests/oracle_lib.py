@@ -19,6 +19,7 @@ assert PARTICLE_DT.itemsize == 56
 
 DUST, ION, ELECTRON = 0, 1, 2
 SPHERE, PROLATE, OBLATE = 0, 1, 2
+RNG_WELL512, RNG_PHILOX = 0, 1
 
 
 class Config(C.Structure):
