@@ -113,6 +113,7 @@ typedef struct potg_stats_t {
     uint64_t n_nodes;    /* tree nodes built (internal + leaves) */
     uint64_t n_particles;
     uint64_t max_depth;
+    uint64_t n_range_loads; /* child ranges fetched by walk wavefronts */
 } potg_stats_t;
 
 typedef struct potg_ctx potg_ctx;

@@ -51,11 +51,12 @@ build_pot() {  # name theta bx by bz [extra -D...]
 
 build_sph() {  # name shape theta bx by bz
     local name=$1 shape=$2 theta=$3 bx=$4 by=$5 bz=$6
-    local extra=()
-    if [ "$shape" = OBLATE ]; then extra=(-e '/^#define PROLATE$/d' ); fi
+    # the shipped file defines PROLATE (SPH:113); OBLATE replaces it
+    local extra=() shape_def=()
+    if [ "$shape" = OBLATE ]; then extra=(-e '/^#define PROLATE$/d'); shape_def=(-DOBLATE); fi
     tr -d '\r' < "$REF/pot-spheroid.c" | sed "${SPH_DEL[@]}" "${extra[@]}" |
         MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
-            -D"$shape" -DOAP="$theta" \
+            "${shape_def[@]}" -DOAP="$theta" \
             -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
             -DRANDOMNESS_SOURCE='"seed.bin"' -o "$OUT/$name" -lm
 }
@@ -65,8 +66,12 @@ build_pot pot_sphere 0.5 0.0 0.0 0.0
 build_pot pot_sphere_bz 0.5 0.0 0.0 50.0
 build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
 build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
+# benchmark workloads (pot_amd/workloads.py): config 2 (B along z, r_ge =
+# lambda_D / 2) and config 3 (prolate, B 30 degrees off the axis)
+build_pot pot_cfg2 0.5 0.0 0.0 3.72363
 # spheroidal grain (pot-spheroid.c; config 3 uses B tilted 30 degrees off the x axis)
 build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
 build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0
+build_sph pot_cfg3 PROLATE 0.5 1.61238 0.0 0.93091
 
 echo "build_ref: built $(ls "$OUT" | wc -l) reference variants in $OUT"

@@ -55,6 +55,7 @@ typedef struct or_node {
     double cx, cy, cz;
     int q;
     unsigned int aq;
+    double px, py, pz; /* dipole about (cx, cy, cz), DIPOLES only */
     unsigned int ch[8];
 } or_node;
 
@@ -674,12 +675,21 @@ static int octant(const or_particle* p, double mid_x, double mid_y, double mid_z
     return (p->z < mid_z) ? GXGYLZ : GXGYGZ;
 }
 
-/* update_node_moments, pot.c:1290-1310 (monopole) */
-static void update_node_moments(or_node* node, const or_node* chi)
+/* update_node_moments, pot.c:1290-1310 (the dipole lines are DIPOLES) */
+static void update_node_moments(or_node* node, const or_node* chi, int dipoles)
 {
+    const double old_cx = node->cx, old_cy = node->cy, old_cz = node->cz;
     node->cx = (node->aq * node->cx + chi->aq * chi->cx) / (node->aq + chi->aq);
     node->cy = (node->aq * node->cy + chi->aq * chi->cy) / (node->aq + chi->aq);
     node->cz = (node->aq * node->cz + chi->aq * chi->cz) / (node->aq + chi->aq);
+    if (dipoles) {
+        node->px -= (node->cx - old_cx) * node->q;
+        node->py -= (node->cy - old_cy) * node->q;
+        node->pz -= (node->cz - old_cz) * node->q;
+        node->px += chi->px - (node->cx - chi->cx) * chi->q;
+        node->py += chi->py - (node->cy - chi->cy) * chi->q;
+        node->pz += chi->pz - (node->cz - chi->cz) * chi->q;
+    }
     node->q += chi->q;
     node->aq += chi->aq;
 }
@@ -696,19 +706,25 @@ static void compute_node_values(or_sim* s, unsigned int node_no)
         node->cx = p->x;
         node->cy = p->y;
         node->cz = p->z;
+        node->px = 0.0;
+        node->py = 0.0;
+        node->pz = 0.0;
     } else {
         node->q = 0;
         node->aq = 0;
         node->cx = 0.0;
         node->cy = 0.0;
         node->cz = 0.0;
+        node->px = 0.0;
+        node->py = 0.0;
+        node->pz = 0.0;
         for (k = 0; k < 8; k++) {
             unsigned int c = s->root[node_no].ch[k];
             if (c) {
                 compute_node_values(s, c);
                 node = &s->root[node_no];
                 if (s->root[c].aq)
-                    update_node_moments(node, &s->root[c]);
+                    update_node_moments(node, &s->root[c], s->ru.dipoles);
             }
         }
     }
@@ -802,6 +818,36 @@ int or_build_tree(or_sim* s)
 }
 
 uint32_t or_tree_used(const or_sim* s) { return s->tr_used; }
+
+int64_t or_tree_canon_dipoles(const or_sim* s, double* p, int64_t cap)
+{
+    unsigned int stack[4096];
+    int sp = 0;
+    int64_t n = 0;
+    if (s->root[0].part == UINT_MAX && s->root[0].ch[0] == 0)
+        return 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const or_node* nd = &s->root[stack[--sp]];
+        int k;
+        if (n >= cap)
+            return -1;
+        p[3 * n] = nd->px;
+        p[3 * n + 1] = nd->py;
+        p[3 * n + 2] = nd->pz;
+        n++;
+        if (nd->part == UINT_MAX && nd->ch[0])
+            for (k = 7; k >= 0; k--) {
+                const or_node* c = &s->root[nd->ch[k]];
+                if (!(c->part == UINT_MAX && c->ch[0] == 0)) {
+                    if (sp >= 4096)
+                        return -2;
+                    stack[sp++] = nd->ch[k];
+                }
+            }
+    }
+    return n;
+}
 
 static int node_empty(const or_node* n) { return n->part == UINT_MAX && n->ch[0] == 0; }
 
@@ -993,13 +1039,21 @@ double or_phi_from_grain(const or_sim* s, uint32_t part)
                 dist2 = (dx * dx) + (dy * dy) + (dz * dz);                                \
                 if (node->ch[0]) st_mac++;                                    \
                 if (((node->x2 - node->x1) * (node->x2 - node->x1)) < (oap2 * dist2)) {   \
-                    if (node->q) {                                                        \
+                    if (ru->dipoles ? (node->aq != 0) : (node->q != 0)) {                 \
                         dist = sqrt(dist2);                                               \
                         E_scale = QE / (4 * PI * EPSILON0 * dist2 * dist);                \
-                        E_scale *= node->q;                                               \
-                        ACC[0] += E_scale * dx;                                           \
-                        ACC[1] += E_scale * dy;                                           \
-                        ACC[2] += E_scale * dz;                                           \
+                        if (ru->dipoles) {                                                \
+                            double rpd2 = (dx * node->px) + (dy * node->py) + (dz * node->pz); \
+                            rpd2 /= dist2;                                                \
+                            ACC[0] += E_scale * ((node->q * dx) - node->px + (3 * dx * rpd2)); \
+                            ACC[1] += E_scale * ((node->q * dy) - node->py + (3 * dy * rpd2)); \
+                            ACC[2] += E_scale * ((node->q * dz) - node->pz + (3 * dz * rpd2)); \
+                        } else {                                                          \
+                            E_scale *= node->q;                                           \
+                            ACC[0] += E_scale * dx;                                       \
+                            ACC[1] += E_scale * dy;                                       \
+                            ACC[2] += E_scale * dz;                                       \
+                        }                                                                 \
                         st_cell++;                                                        \
                     }                                                                     \
                 } else {                                                                  \
@@ -1143,8 +1197,13 @@ double or_phi_at(or_sim* s, double x, double y, double z)
             dz = z - node->cz;
             dist2 = (dx * dx) + (dy * dy) + (dz * dz);
             if (((node->x2 - node->x1) * (node->x2 - node->x1)) < (oap2 * dist2)) {
-                if (node->q)
+                if (ru->dipoles && node->aq) {
+                    double r_dot_p = (dx * node->px) + (dy * node->py) + (dz * node->pz);
+                    dist = sqrt(dist2);
+                    phi += (QE / (4 * PI * EPSILON0)) * ((node->q / dist) + (r_dot_p / (dist * dist2)));
+                } else if (!ru->dipoles && node->q) {
                     phi += QE * node->q / (4 * PI * EPSILON0 * sqrt(dist2));
+                }
                 path_len--;
             } else {
                 int k;
@@ -1186,8 +1245,13 @@ static long double phi_particle_feels_ld(or_sim* s, uint32_t part)
             dz = s->parts[part].z - node->cz;
             dist2 = (dx * dx) + (dy * dy) + (dz * dz);
             if (((node->x2 - node->x1) * (node->x2 - node->x1)) < (oap2 * dist2)) {
-                if (node->q)
+                if (ru->dipoles && node->aq) {
+                    double r_dot_p = (dx * node->px) + (dy * node->py) + (dz * node->pz);
+                    dist = sqrt(dist2);
+                    phi += QE * ((node->q / dist) + (r_dot_p / (dist * dist2))) / (4 * PI * EPSILON0);
+                } else if (!ru->dipoles && node->q) {
                     phi += QE * node->q / (4 * PI * EPSILON0 * sqrt(dist2));
+                }
             } else {
                 int k;
                 if ((path_len + 8) > MAX_PATH_LEN)

@@ -66,7 +66,7 @@ typedef struct or_config {
     int32_t rng_kind;        /* OR_RNG_* */
     uint64_t philox_seed;    /* key of the counter-based stream */
     int32_t record_macro;    /* RECORD_MACRO_DATA switch (drives compute_epe cadence) */
-    int32_t pad0;
+    int32_t dipoles;         /* DIPOLES switch (pot.c:99-105): cell dipole moments */
 } or_config;
 
 /* Per-walk interaction counters. */
@@ -124,6 +124,8 @@ int or_build_tree(or_sim* s);
 int64_t or_tree_canon(const or_sim* s, int32_t* depth, int32_t* is_leaf,
                       uint32_t* part, int32_t* q, uint32_t* aq, double* c,
                       double* x1x2, int32_t* nchild, int64_t cap);
+/* dipole moments of the canonical listing (zeros without DIPOLES) */
+int64_t or_tree_canon_dipoles(const or_sim* s, double* p, int64_t cap);
 uint32_t or_tree_used(const or_sim* s);
 
 /* walks (pot.c:1696-1808, 1818-1901, 1934-2018; SPH:1689-1780, 1922-2084) */

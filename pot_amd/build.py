@@ -61,19 +61,28 @@ def build_lib(jobs=8):
     return LIB
 
 
+HOST_LIB_SOURCES = ["pot_init.c"]
+HOST_MAIN = "pot_main.c"
+HOST_LIB = os.path.join(BUILD, "libpothost.so")
+
+
 def build_host():
-    """The drop-in host programs (C): pot and pot-spheroid over libpotg."""
-    srcs = [os.path.join(HOST, f) for f in sorted(os.listdir(HOST)) if f.endswith(".c")] \
-        if os.path.isdir(HOST) else []
-    if not srcs:
-        return []
-    outs = []
-    for name, geom in (("pot", "SPHERE"), ("pot-spheroid", "SPHEROID")):
-        out = os.path.join(BUILD, name)
-        _run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-D_GNU_SOURCE",
-              "-DPOT_GEOMETRY_" + geom, "-I" + os.path.join(REPO, "include")] + srcs +
-             ["-L" + BUILD, "-lpotg", "-Wl,-rpath,$ORIGIN", "-lm", "-o", out])
-        outs.append(out)
+    """The host side in C: libpothost.so (initial load, state files, outputs)
+    and, when its main exists, the drop-in programs pot and pot-spheroid."""
+    os.makedirs(BUILD, exist_ok=True)
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + HOST]
+    cflags = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-D_GNU_SOURCE", "-fPIC"]
+    srcs = [os.path.join(HOST, f) for f in HOST_LIB_SOURCES]
+    _run(cflags + ["-shared"] + inc + srcs + ["-L" + BUILD, "-lpotg", "-Wl,-rpath,$ORIGIN",
+                                             "-lm", "-o", HOST_LIB])
+    outs = [HOST_LIB]
+    main = os.path.join(HOST, HOST_MAIN)
+    if os.path.exists(main):
+        for name, geom in (("pot", "SPHERE"), ("pot-spheroid", "SPHEROID")):
+            out = os.path.join(BUILD, name)
+            _run(cflags + ["-DPOT_GEOMETRY_" + geom] + inc + [main] + srcs +
+                 ["-L" + BUILD, "-lpotg", "-Wl,-rpath,$ORIGIN", "-lm", "-o", out])
+            outs.append(out)
     return outs
 
 

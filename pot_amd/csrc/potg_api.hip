@@ -43,7 +43,7 @@ struct potg_ctx {
     Bufs b;
     std::vector<potg_config> cfg;
     std::vector<InstParams> hp;   // host copy of the parameters
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;
     hipEvent_t ev[6];
     int timing;
     double stage_ms[5];
@@ -66,7 +66,7 @@ int dev_alloc(potg_ctx* c, void** p, size_t bytes)
         set_err("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
         return POTG_ERR_NOMEM;
     }
-    c->allocs.push_back(*p);
+    c->allocs.push_back({*p, bytes});
     return POTG_OK;
 }
 
@@ -335,9 +335,12 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
         potg_destroy(c);
         return POTG_ERR_NOMEM;
     }
-    // zeroed records: a record the build has not written is an empty cell
-    if (hipMemset(b.rec64, 0, sizeof(NodeRec64) * b.n_rec) != hipSuccess ||
-        (b.rec32 && hipMemset(b.rec32, 0, sizeof(NodeRec32) * b.n_rec) != hipSuccess)) {
+    // zeroed tree arrays: an entry the build has not written yet is an empty
+    // cell or index 0, never an address outside the buffers
+    bool clear_ok = true;
+    for (size_t k = 0; k < c->allocs.size(); k++)
+        if (hipMemset(c->allocs[k].first, 0, c->allocs[k].second) != hipSuccess) clear_ok = false;
+    if (!clear_ok) {
         set_err("initial clear failed");
         potg_destroy(c);
         return POTG_ERR_HIP;
@@ -355,6 +358,12 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
         potg_destroy(c);
         return POTG_ERR_HIP;
     }
+    launch_init_slots(b, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_err("slot initialisation failed");
+        potg_destroy(c);
+        return POTG_ERR_HIP;
+    }
     *out = c;
     return POTG_OK;
 }
@@ -364,7 +373,7 @@ int potg_destroy(potg_ctx* c)
     if (!c) return POTG_OK;
     (void)hipStreamSynchronize(c->stream);
     destroy_graphs(c);
-    for (void* p : c->allocs) (void)hipFree(p);
+    for (auto& a : c->allocs) (void)hipFree(a.first);
     if (c->h_aos) (void)hipHostFree(c->h_aos);
     if (c->d_aos) (void)hipFree(c->d_aos);
     for (int k = 0; k < 6; k++)
@@ -401,6 +410,8 @@ int potg_upload(potg_ctx* c, uint32_t inst, const potg_particle* parts, const po
     HIPCHK(hipMemcpyAsync(c->d_aos, c->h_aos, N * sizeof(potg_particle), hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipMemcpyAsync(&c->b.P[inst], &p, sizeof(InstParams), hipMemcpyHostToDevice, c->stream));
+    // reference layout first, so the instance's slots are [offset, offset + N)
+    launch_unsort(c->b, c->stream);
     launch_scatter_upload(c->b, inst, (const double*)c->d_aos, (uint32_t)N, c->stream);
     launch_build_inverse(c->b, c->stream);
     InstState s;
@@ -655,6 +666,7 @@ int potg_stats(potg_ctx* c, potg_stats_t* out)
     out->n_nodes = (uint64_t)g.n_internal + c->b.n_tree;
     out->n_particles = c->b.n_total;
     out->max_depth = g.max_depth;
+    out->n_range_loads = g.n_pops;
     return POTG_OK;
 }
 
@@ -663,7 +675,7 @@ int potg_stats_reset(potg_ctx* c)
     if (!c) return set_err("potg_stats_reset: null context"), POTG_ERR_ARG;
     Globals g;
     HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
-    g.n_cell = g.n_leaf = g.n_open = g.n_tgt_total = g.n_steps = 0;
+    g.n_cell = g.n_leaf = g.n_open = g.n_tgt_total = g.n_steps = g.n_pops = 0;
     g.max_depth = 0;
     HIPCHK(hipMemcpy(c->b.g, &g, sizeof(g), hipMemcpyHostToDevice));
     return POTG_OK;

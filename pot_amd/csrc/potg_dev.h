@@ -96,6 +96,7 @@ struct Globals {
     uint32_t err_idx;
     uint32_t max_depth;
     unsigned long long n_cell, n_leaf, n_open, n_tgt_total, n_steps;
+    unsigned long long n_pops;   // child ranges fetched by the walk (per wavefront)
 };
 
 // Tree node records, children contiguous: an internal node's children are

@@ -80,6 +80,10 @@ void launch_epe(Bufs& b, hipStream_t s);
 void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n, hipStream_t s);
 void launch_gather_download(Bufs& b, uint32_t inst, double* aos, uint32_t n, hipStream_t s);
 void launch_build_inverse(Bufs& b, hipStream_t s);
+// set every slot to its reference identity (at creation)
+void launch_init_slots(Bufs& b, hipStream_t s);
+// permute the state back to reference layout (before an upload)
+void launch_unsort(Bufs& b, hipStream_t s);
 // point queries: n points of q_pts with q_inst/q_self; kind 0 field, 1 potential
 void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s);
 void launch_direct_sum(Bufs& b, uint32_t inst, uint32_t n, hipStream_t s);

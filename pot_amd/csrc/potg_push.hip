@@ -473,7 +473,45 @@ __global__ void k_download(PState st, const uint32_t* __restrict__ inv, AosParti
 __global__ void k_inverse(PState st, uint32_t n, uint32_t* __restrict__ inv)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inv[st.ref[i]] = i;
+    if (i >= n) return;
+    const uint32_t r = st.ref[i];
+    if (r < n) inv[r] = i;
+}
+
+// every slot starts as itself: reference layout, owned by its instance
+__global__ void k_init_slots(PState st, const InstParams* __restrict__ P, uint32_t n_inst,
+                             uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = n_inst - 1;
+    while (lo < hi) {  // last instance whose offset <= i
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (P[mid].offset <= i) lo = mid; else hi = mid - 1;
+    }
+    st.x[i] = st.y[i] = st.z[i] = 0.0;
+    st.vx[i] = st.vy[i] = st.vz[i] = 0.0;
+    st.ref[i] = i;
+    st.inst[i] = (uint16_t)lo;
+    st.spec[i] = (i == P[lo].offset) ? 0 : 2;
+}
+
+// back to reference layout: slot r holds the particle whose reference slot is r
+__global__ void k_unsort(PState src, PState dst, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = src.ref[i];
+    if (r >= n) return;
+    dst.x[r] = src.x[i];
+    dst.y[r] = src.y[i];
+    dst.z[r] = src.z[i];
+    dst.vx[r] = src.vx[i];
+    dst.vy[r] = src.vy[i];
+    dst.vz[r] = src.vz[i];
+    dst.ref[r] = r;
+    dst.inst[r] = src.inst[i];
+    dst.spec[r] = src.spec[i];
 }
 
 // ---- float64 direct sum (the force oracle on the GPU) --------------------
@@ -576,6 +614,20 @@ void launch_gather_download(Bufs& b, uint32_t inst, double* aos, uint32_t n, hip
 void launch_build_inverse(Bufs& b, hipStream_t s)
 {
     k_inverse<<<grid(b.n_total), BT, 0, s>>>(b.st, b.n_total, b.inv);
+}
+
+void launch_init_slots(Bufs& b, hipStream_t s)
+{
+    k_init_slots<<<grid(b.n_total), BT, 0, s>>>(b.st, b.P, b.n_inst, b.n_total);
+    k_inverse<<<grid(b.n_total), BT, 0, s>>>(b.st, b.n_total, b.inv);
+}
+
+void launch_unsort(Bufs& b, hipStream_t s)
+{
+    k_unsort<<<grid(b.n_total), BT, 0, s>>>(b.st, b.alt, b.n_total);
+    PState t = b.st;
+    b.st = b.alt;
+    b.alt = t;
 }
 
 void launch_direct_sum(Bufs& b, uint32_t inst, uint32_t n, hipStream_t s)

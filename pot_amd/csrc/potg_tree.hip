@@ -177,7 +177,6 @@ __global__ void k_lcp(const uint64_t* __restrict__ key, PState st, const InstPar
                       uint32_t* __restrict__ cnt, Globals* g)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_tree) return;
     int v = -1;
     if (i + 1 < n_tree && st.inst[i] == st.inst[i + 1]) {
         const uint64_t dmask = (levels >= 21) ? ~0ull >> 1 : ((1ull << (3 * levels)) - 1);
@@ -191,8 +190,14 @@ __global__ void k_lcp(const uint64_t* __restrict__ key, PState st, const InstPar
             if (v >= MAXD - 1) raise_err(g, KERR_COINCIDENT, st.ref[i]);
         }
     }
-    lcp[i] = v;
-    atomicMax(&g->max_depth, (uint32_t)(v + 2));
+    if (i < n_tree) lcp[i] = v;
+    // deepest internal level + 1, one atomic per wavefront
+    uint32_t m = (uint32_t)(v + 1);
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t o = __shfl_xor(m, d, WAVE);
+        m = o > m ? o : m;
+    }
+    if (__lane_id() == 0) atomicMax(&g->max_depth, m);
 }
 
 __global__ void k_counts(const int32_t* __restrict__ lcp, uint32_t n_tree, uint32_t* __restrict__ cnt)
@@ -408,92 +413,104 @@ __device__ inline void put32(NodeRec32* r32, uint32_t w, const InstParams& p, do
     r32[w] = f;
 }
 
-// bottom-up monopole moments: the last child to arrive folds its parent
-__global__ void k_moments(PState st, const InstParams* __restrict__ P, const InstState* __restrict__ S,
-                          uint32_t n_tree, const uint32_t* __restrict__ slots,
-                          const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ cfirst,
-                          const uint32_t* __restrict__ par_int, const uint32_t* __restrict__ par_leaf,
-                          const uint32_t* __restrict__ wid_int, const uint32_t* __restrict__ wid_leaf,
-                          const double* __restrict__ size2, uint32_t* arrive, double* mcx,
-                          double* mcy, double* mcz, int32_t* mq, uint32_t* maq,
-                          NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32, uint32_t cap)
+// leaf records: every tree particle is a leaf of charge CHARGE[species]
+// (the grain: its current charge, pot.c:1318-1328)
+__global__ void k_leaf_records(PState st, const InstParams* __restrict__ P,
+                               const InstState* __restrict__ S, uint32_t n_tree, uint32_t n_rec,
+                               const uint32_t* __restrict__ wid_leaf, NodeRec64* __restrict__ r64,
+                               NodeRec32* __restrict__ r32)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_tree) return;
     const uint32_t inst = st.inst[i];
-    const InstParams& p = P[inst];
+    NodeRec64 e;
+    e.cx = st.x[i];
+    e.cy = st.y[i];
+    e.cz = st.z[i];
+    e.size2 = 0.0;
+    e.q = charge_of(st.spec[i], S[inst].grain_q);
+    e.first = 0;
+    e.cnt = LEAF;
+    e.part = i;
+    const uint32_t w = wid_leaf[i];
+    if (w >= n_rec) return;
+    r64[w] = e;
+    if (r32) put32(r32, w, P[inst], e.cx, e.cy, e.cz, 0.0, e.q, 0, LEAF, i);
+}
+
+// monopole moments of the internal nodes at one depth, children folded in
+// the reference order (compute_node_values / update_node_moments,
+// pot.c:1290-1402); the deeper levels were finished by earlier launches
+__global__ void k_level_moments(int L, PState st, const InstParams* __restrict__ P,
+                                const InstState* __restrict__ S, const Globals* __restrict__ g,
+                                uint32_t cap, const int32_t* __restrict__ level,
+                                const uint32_t* __restrict__ owner,
+                                const uint32_t* __restrict__ slots,
+                                const uint32_t* __restrict__ ccount,
+                                const uint32_t* __restrict__ cfirst,
+                                const uint32_t* __restrict__ wid_int,
+                                const double* __restrict__ size2, double* __restrict__ mcx,
+                                double* __restrict__ mcy, double* __restrict__ mcz,
+                                int32_t* __restrict__ mq, uint32_t* __restrict__ maq,
+                                NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
+                                uint32_t n_rec)
+{
+    if ((uint32_t)L + 1 > g->max_depth) return;  // no internal node this deep
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_int_of(g, cap) || level[k] != L) return;
+    const uint32_t inst = st.inst[owner[k]];
     const int32_t gq = S[inst].grain_q;
-    {
-        NodeRec64 e;
-        e.cx = st.x[i];
-        e.cy = st.y[i];
-        e.cz = st.z[i];
-        e.size2 = 0.0;
-        e.q = charge_of(st.spec[i], gq);
-        e.first = 0;
-        e.cnt = LEAF;
-        e.part = i;
-        r64[wid_leaf[i]] = e;
-        if (r32) put32(r32, wid_leaf[i], p, e.cx, e.cy, e.cz, 0.0, e.q, 0, LEAF, i);
-    }
-    uint32_t par = par_leaf[i];
-    while (par != NONE && par < cap) {
-        __threadfence();
-        const uint32_t old = atomicAdd(&arrive[par], 1u);
-        if (old + 1 != ccount[par]) return;
-        __threadfence();
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        int q = 0;
-        uint32_t aq = 0;
-        for (int s = 0; s < 8; s++) {
-            const uint32_t c = slots[8 * par + s];
-            if (c == NONE) continue;
-            int qc;
-            uint32_t aqc;
-            double ccx, ccy, ccz;
-            if (c & LEAF_BIT) {
-                const uint32_t j = c & ~LEAF_BIT;
-                qc = charge_of(st.spec[j], gq);
-                aqc = (uint32_t)abs(qc);
-                ccx = st.x[j];
-                ccy = st.y[j];
-                ccz = st.z[j];
-            } else {
-                // written by another thread of this kernel: coherent loads
-                qc = __hip_atomic_load(&mq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                aqc = __hip_atomic_load(&maq[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ccx = __hip_atomic_load(&mcx[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ccy = __hip_atomic_load(&mcy[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ccz = __hip_atomic_load(&mcz[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (aqc) {
-                // update_node_moments (pot.c:1297-1299): fold in child order
-                const double den = (double)(aq + aqc);
-                cx = ((double)aq * cx + (double)aqc * ccx) / den;
-                cy = ((double)aq * cy + (double)aqc * ccy) / den;
-                cz = ((double)aq * cz + (double)aqc * ccz) / den;
-                q += qc;
-                aq += aqc;
-            }
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    int q = 0;
+    uint32_t aq = 0;
+    for (int s = 0; s < 8; s++) {
+        const uint32_t c = slots[8 * k + s];
+        if (c == NONE) continue;
+        int qc;
+        uint32_t aqc;
+        double ccx, ccy, ccz;
+        if (c & LEAF_BIT) {
+            const uint32_t j = c & ~LEAF_BIT;
+            qc = charge_of(st.spec[j], gq);
+            aqc = (uint32_t)abs(qc);
+            ccx = st.x[j];
+            ccy = st.y[j];
+            ccz = st.z[j];
+        } else {
+            qc = mq[c];
+            aqc = maq[c];
+            ccx = mcx[c];
+            ccy = mcy[c];
+            ccz = mcz[c];
         }
-        __hip_atomic_store(&mcx[par], cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&mcy[par], cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&mcz[par], cz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&mq[par], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&maq[par], aq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        NodeRec64 e;
-        e.cx = cx;
-        e.cy = cy;
-        e.cz = cz;
-        e.size2 = size2[par];
-        e.q = q;
-        e.first = cfirst[par];
-        e.cnt = ccount[par];
-        e.part = aq;
-        r64[wid_int[par]] = e;
-        if (r32) put32(r32, wid_int[par], p, cx, cy, cz, e.size2, q, e.first, e.cnt, aq);
-        par = par_int[par];
+        if (aqc) {
+            // update_node_moments (pot.c:1297-1299), same float64 operations
+            const double den = (double)(aq + aqc);
+            cx = ((double)aq * cx + (double)aqc * ccx) / den;
+            cy = ((double)aq * cy + (double)aqc * ccy) / den;
+            cz = ((double)aq * cz + (double)aqc * ccz) / den;
+            q += qc;
+            aq += aqc;
+        }
     }
+    mcx[k] = cx;
+    mcy[k] = cy;
+    mcz[k] = cz;
+    mq[k] = q;
+    maq[k] = aq;
+    NodeRec64 e;
+    e.cx = cx;
+    e.cy = cy;
+    e.cz = cz;
+    e.size2 = size2[k];
+    e.q = q;
+    e.first = cfirst[k];
+    e.cnt = ccount[k];
+    e.part = aq;
+    const uint32_t w = wid_int[k];
+    if (w >= n_rec) return;
+    r64[w] = e;
+    if (r32) put32(r32, w, P[inst], cx, cy, cz, e.size2, q, e.first, e.cnt, aq);
 }
 
 }  // namespace
@@ -534,11 +551,18 @@ void launch_tree(Bufs& b, hipStream_t s)
         k_wids<<<grid(cap), BT, 0, s>>>(b.slots, b.cfirst, b.g, cap, b.wid_int, b.wid_leaf);
     }
     k_empty_roots<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.n_inst, b.rec64, b.precision ? b.rec32 : nullptr);
-    if (n)
-        k_moments<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.slots, b.ccount, b.cfirst, b.par_int,
-                                         b.par_leaf, b.wid_int, b.wid_leaf, b.size2, b.arrive,
-                                         b.mcx, b.mcy, b.mcz, b.mq, b.maq, b.rec64,
-                                         b.precision ? b.rec32 : nullptr, b.cap_int);
+    if (n) {
+        NodeRec32* r32 = b.precision ? b.rec32 : nullptr;
+        k_leaf_records<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.n_rec, b.wid_leaf, b.rec64,
+                                              r32);
+        // deepest level first; launches past the tree's depth return at once
+        for (int L = MAXD - 1; L >= 0; L--)
+            k_level_moments<<<grid(b.cap_int), BT, 0, s>>>(L, b.st, b.P, b.S, b.g, b.cap_int,
+                                                           b.level, b.owner, b.slots, b.ccount,
+                                                           b.cfirst, b.wid_int, b.size2, b.mcx,
+                                                           b.mcy, b.mcz, b.mq, b.maq, b.rec64, r32,
+                                                           b.n_rec);
+    }
 }
 
 }  // namespace potg

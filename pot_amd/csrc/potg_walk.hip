@@ -4,18 +4,23 @@
 // sort order.  The walk keeps the reference's per-particle opening test
 // (field_particle_feels_at, pot.c:1718-1806): the wave shares one
 // depth-first traversal whose stack entries carry a 64-bit lane mask; at a
-// popped cell every lane in the mask applies its own MAC, lanes that accept
-// it add the monopole and lanes that reject it pass the children on with
-// their mask.  Each lane therefore visits exactly the nodes, in exactly the
-// order, that the reference visits for that particle.  Stack entries are
-// child ranges (children of a node are contiguous records), so the stack
-// depth is the tree depth; it lives in LDS, the popped node index is
-// wave-uniform and node records come through the scalar cache.
+// cell every lane in the mask applies its own MAC, lanes that accept it add
+// the monopole, and lanes that reject it pass the children on with their
+// mask.  Each lane therefore meets exactly the nodes, in exactly the order,
+// that the reference visits for that particle.
 //
-// FP64: every operation matches the reference's float64 expressions, with
+// Memory: the children of a node are contiguous records, so a stack entry
+// is a child range.  A range is fetched with one vector load (lane j holds
+// child j) and its children are then visited from registers, broadcast with
+// v_readlane -- one memory round trip per opened node instead of one per
+// child.  Ranges interrupted by a descent are pushed back (remaining
+// siblings, same mask) on an LDS stack whose depth is the tree depth.
+//
+// FP64: every operation repeats the reference's float64 expressions, with
 // float64 instead of x87 accumulation.  MIXED: coordinates relative to the
-// domain centre in units of R, float32 arithmetic, Kahan-compensated
-// per-particle sums, and the result scaled back in float64.
+// domain centre in units of R, float32 arithmetic with a hardware reciprocal
+// square root, Kahan-compensated per-particle sums, result scaled back in
+// float64.
 #include "potg_grain.h"
 #include "potg_internal.h"
 
@@ -37,16 +42,11 @@ struct WalkIO {
     const uint32_t* pt_inst;
     const uint32_t* pt_self;
     uint32_t n_pts;
-    double* out;
+    double* out;       // in: grain term (initial accumulator); out: result
     uint32_t* counts;
     uint32_t* tile_ctr;
-    uint32_t n_rec;   // record capacity: child ranges must stay inside it
+    uint32_t n_rec;    // record capacity: child ranges must stay inside it
 };
-
-__device__ inline int charge_of(uint8_t s, int32_t grain_q)
-{
-    return s == 0 ? grain_q : (s == 1 ? 1 : -1);
-}
 
 __device__ inline void kahan_add(float& s, float& c, float v)
 {
@@ -56,19 +56,63 @@ __device__ inline void kahan_add(float& s, float& c, float v)
     s = t;
 }
 
-template <int PREC, int MODE, int KIND>
-__global__ __launch_bounds__(BT) void k_walk(const InstParams* __restrict__ P,
-                                              const InstState* __restrict__ S, Globals* g,
-                                              PState st, const NodeRec64* __restrict__ r64,
-                                              const NodeRec32* __restrict__ r32, WalkIO io)
+__device__ inline float bcast_f(float v, uint32_t j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline uint32_t bcast_u(uint32_t v, uint32_t j)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+__device__ inline double bcast_d(double v, uint32_t j)
+{
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// target of lane k: the half-drifted particle (pot.c:2478-2480) or a point
+template <int MODE>
+__device__ inline bool load_target(const InstParams* __restrict__ P, PState st, const WalkIO& io,
+                                   uint32_t k, uint32_t ntg, uint32_t& inst, uint32_t& self,
+                                   double& tx, double& ty, double& tz)
+{
+    inst = 0;
+    self = NONE;
+    tx = ty = tz = 0.0;
+    if (k >= ntg) return false;
+    if (MODE == MODE_STEP) {
+        const uint32_t i = io.targets[k];
+        self = i;
+        inst = st.inst[i];
+        const double dts = P[inst].dt * (double)P[inst].tss[st.spec[i]];
+        tx = st.x[i] + (0.5 * dts) * st.vx[i];
+        ty = st.y[i] + (0.5 * dts) * st.vy[i];
+        tz = st.z[i] + (0.5 * dts) * st.vz[i];
+    } else {
+        inst = io.pt_inst[k];
+        self = io.pt_self[k];
+        tx = io.pts[3 * k];
+        ty = io.pts[3 * k + 1];
+        tz = io.pts[3 * k + 2];
+    }
+    return true;
+}
+
+// ---- float64 walk (field or potential) --------------------------------
+template <int MODE, int KIND>
+__global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P, Globals* g,
+                                                PState st, const NodeRec64* __restrict__ rec,
+                                                WalkIO io)
 {
     __shared__ uint32_t s_first[NW][STK];
     __shared__ uint32_t s_cnt[NW][STK];
     __shared__ uint64_t s_mask[NW][STK];
     const int wv = threadIdx.x / WAVE;
-    const int lane = __lane_id();
+    const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
-    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0;
+    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
 
     for (;;) {
         uint32_t tile = 0;
@@ -76,55 +120,21 @@ __global__ __launch_bounds__(BT) void k_walk(const InstParams* __restrict__ P,
         tile = rfl(__shfl(tile, 0, WAVE));
         if ((uint64_t)tile * WAVE >= ntg) break;
         const uint32_t k = tile * WAVE + lane;
-        const bool valid = k < ntg;
-
-        uint32_t inst = 0, self = NONE;
-        double tx = 0.0, ty = 0.0, tz = 0.0;
-        if (valid) {
-            if (MODE == MODE_STEP) {
-                const uint32_t i = io.targets[k];
-                self = i;
-                inst = st.inst[i];
-                const double dts = P[inst].dt * (double)P[inst].tss[st.spec[i]];
-                // half drift (pot.c:2478-2480)
-                tx = st.x[i] + (0.5 * dts) * st.vx[i];
-                ty = st.y[i] + (0.5 * dts) * st.vy[i];
-                tz = st.z[i] + (0.5 * dts) * st.vz[i];
-            } else {
-                inst = io.pt_inst[k];
-                self = io.pt_self[k];
-                tx = io.pts[3 * k];
-                ty = io.pts[3 * k + 1];
-                tz = io.pts[3 * k + 2];
-            }
-        }
-        const InstParams& p = P[inst];
-        const double theta2 = p.theta2, soft2 = p.soft2;
-        const int32_t gq = S[inst].grain_q;
-
-        // accumulators
+        uint32_t inst, self;
+        double tx, ty, tz;
+        const bool valid = load_target<MODE>(P, st, io, k, ntg, inst, self, tx, ty, tz);
+        const double theta2 = P[inst].theta2, soft2 = P[inst].soft2;
         double ex = 0.0, ey = 0.0, ez = 0.0, ph = 0.0;
-        float fx = 0.f, fy = 0.f, fz = 0.f, cx_ = 0.f, cy_ = 0.f, cz_ = 0.f;
-        float ftx = 0.f, fty = 0.f, ftz = 0.f, ftheta2 = 0.f, fsoft2 = 0.f;
-        uint32_t ncell = 0, nleaf = 0, nopen = 0;
-        if (valid && p.geometry != 0) {
+        if (valid) {
             if (KIND == KIND_FIELD) {
-                double e3[3];
-                grain_field(p, gq, tx, ty, tz, e3);
-                ex = e3[0];
-                ey = e3[1];
-                ez = e3[2];
+                ex = io.out[3 * k];
+                ey = io.out[3 * k + 1];
+                ez = io.out[3 * k + 2];
             } else {
-                ph = grain_phi_abs(p, gq, tx, ty, tz);
+                ph = io.out[k];
             }
         }
-        if (PREC == 32) {
-            ftx = (float)((tx - p.R) * p.invR);
-            fty = (float)((ty - p.R) * p.invR);
-            ftz = (float)((tz - p.R) * p.invR);
-            ftheta2 = (float)theta2;
-            fsoft2 = (float)(soft2 * p.invR * p.invR);
-        }
+        uint32_t ncell = 0, nleaf = 0, nopen = 0;
 
         uint64_t pending = ballot(valid);
         while (pending) {
@@ -132,16 +142,11 @@ __global__ __launch_bounds__(BT) void k_walk(const InstParams* __restrict__ P,
             const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
-
             uint32_t tf = gi, tc = 1;
             uint64_t tm = grp;
             int sp = 0;
-            uint32_t pops = 0;
+            uint32_t loads = 0;
             for (;;) {
-                if (++pops > io.n_rec) {
-                    raise_err(g, KERR_DEPTH, tf);
-                    break;
-                }
                 if (tc == 0) {
                     if (sp == 0) break;
                     --sp;
@@ -149,118 +154,100 @@ __global__ __launch_bounds__(BT) void k_walk(const InstParams* __restrict__ P,
                     tc = rfl(s_cnt[wv][sp]);
                     tm = rfl64(s_mask[wv][sp]);
                 }
-                const uint32_t w = tf + tc - 1;
-                tc--;
+                if (++loads > io.n_rec) {
+                    raise_err(g, KERR_DEPTH, tf);
+                    break;
+                }
+                tot_pops++;
+                // one load for the whole range: lane j holds child j
+                NodeRec64 mine;
+                if (lane < tc) mine = rec[tf + lane];
+                else mine = NodeRec64{0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0};
                 const bool act = (tm >> lane) & 1ull;
-                bool open = false;
-                uint32_t rfirst, rcnt;
-                if (PREC == 64 || KIND == KIND_PHI) {
-                    const NodeRec64 r = r64[w];
-                    rfirst = r.first;
-                    rcnt = r.cnt;
+                uint32_t next_first = 0, next_cnt = 0;
+                uint64_t next_mask = 0;
+                int j = (int)tc - 1;
+                for (; j >= 0; j--) {
+                    const double cx = bcast_d(mine.cx, j), cy = bcast_d(mine.cy, j),
+                                 cz = bcast_d(mine.cz, j);
+                    const uint32_t cnt = bcast_u(mine.cnt, j);
+                    const int q = (int)bcast_u((uint32_t)mine.q, j);
+                    bool open = false;
                     if (act) {
-                        const double dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+                        const double dx = tx - cx, dy = ty - cy, dz = tz - cz;
                         const double d2 = (dx * dx) + (dy * dy) + (dz * dz);
-                        if (r.cnt == LEAF) {
+                        if (cnt == LEAF) {
                             if (KIND == KIND_FIELD) {
-                                if (r.part != self) {
+                                if (bcast_u(mine.part, j) != self) {
                                     if (d2 != 0.0) {
                                         const double dd2 = d2 + soft2;
                                         const double dist = sqrt(dd2);
-                                        const double es = (QE * r.q) / ((K4PE0 * dd2) * dist);
+                                        const double es = (QE * q) / ((K4PE0 * dd2) * dist);
                                         ex += es * dx;
                                         ey += es * dy;
                                         ez += es * dz;
                                         nleaf++;
                                     } else {
-                                        raise_err(g, KERR_COINCIDENT, r.part);
+                                        raise_err(g, KERR_COINCIDENT, self);
                                     }
                                 }
                             } else {
-                                ph += (QE * r.q) / (K4PE0 * sqrt(d2 + soft2));
+                                ph += (QE * q) / (K4PE0 * sqrt(d2 + soft2));
                                 nleaf++;
                             }
-                        } else if (r.size2 < theta2 * d2) {
-                            if (r.q) {
+                        } else if (bcast_d(mine.size2, j) < theta2 * d2) {
+                            if (q) {
                                 if (KIND == KIND_FIELD) {
                                     const double dist = sqrt(d2);
                                     double es = QE / ((K4PE0 * d2) * dist);
-                                    es *= r.q;
+                                    es *= q;
                                     ex += es * dx;
                                     ey += es * dy;
                                     ez += es * dz;
                                 } else {
-                                    ph += (QE * r.q) / (K4PE0 * sqrt(d2));
+                                    ph += (QE * q) / (K4PE0 * sqrt(d2));
                                 }
                                 ncell++;
                             }
                         } else {
-                            open = true;
+                            open = cnt != 0;
                         }
                     }
-                } else {
-                    const NodeRec32 r = r32[w];
-                    rfirst = r.first;
-                    rcnt = r.cnt;
-                    if (act) {
-                        const float dx = ftx - r.cx, dy = fty - r.cy, dz = ftz - r.cz;
-                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        if (r.cnt == LEAF) {
-                            if (r.part != self && d2 != 0.f) {
-                                const float inv = __builtin_amdgcn_rsqf(d2 + fsoft2);
-                                const float s = (float)r.q * (inv * inv * inv);
-                                kahan_add(fx, cx_, s * dx);
-                                kahan_add(fy, cy_, s * dy);
-                                kahan_add(fz, cz_, s * dz);
-                                nleaf++;
-                            }
-                        } else if (r.size2 < ftheta2 * d2) {
-                            if (r.q) {
-                                const float inv = __builtin_amdgcn_rsqf(d2);
-                                const float s = (float)r.q * (inv * inv * inv);
-                                kahan_add(fx, cx_, s * dx);
-                                kahan_add(fy, cy_, s * dy);
-                                kahan_add(fz, cz_, s * dz);
-                                ncell++;
-                            }
-                        } else {
-                            open = true;
-                        }
-                    }
-                }
-                const uint64_t om = rfl64(ballot(open));
-                if (om && rcnt != LEAF && rcnt) {
-                    // children lie after their parent and inside the records
-                    if (rfirst <= w || rfirst + rcnt > io.n_rec) {
-                        raise_err(g, KERR_CAPACITY, w);
+                    const uint64_t om = rfl64(ballot(open));
+                    if (om) {
+                        if (open) nopen++;
+                        next_first = bcast_u(mine.first, j);
+                        next_cnt = cnt;
+                        next_mask = om;
                         break;
                     }
-                    if (open) nopen++;
-                    if (tc) {
-                        if (sp >= STK) {
-                            raise_err(g, KERR_DEPTH, w);
-                            break;
-                        }
-                        s_first[wv][sp] = tf;
-                        s_cnt[wv][sp] = tc;
-                        s_mask[wv][sp] = tm;
-                        sp++;
-                    }
-                    tf = rfirst;
-                    tc = rcnt;
-                    tm = om;
                 }
+                if (j < 0) {
+                    tc = 0;
+                    continue;
+                }
+                // descend into child j; its elder siblings wait on the stack
+                if (next_first <= tf + (uint32_t)j || next_first + next_cnt > io.n_rec) {
+                    raise_err(g, KERR_CAPACITY, tf + j);
+                    break;
+                }
+                if (j > 0) {
+                    if (sp >= STK) {
+                        raise_err(g, KERR_DEPTH, tf);
+                        break;
+                    }
+                    s_first[wv][sp] = tf;
+                    s_cnt[wv][sp] = (uint32_t)j;
+                    s_mask[wv][sp] = tm;
+                    sp++;
+                }
+                tf = next_first;
+                tc = next_cnt;
+                tm = next_mask;
             }
         }
-
         if (valid) {
             if (KIND == KIND_FIELD) {
-                if (PREC == 32) {
-                    const double K = QE / (K4PE0 * p.R * p.R);
-                    ex += K * (double)fx;
-                    ey += K * (double)fy;
-                    ez += K * (double)fz;
-                }
                 io.out[3 * k] = ex;
                 io.out[3 * k + 1] = ey;
                 io.out[3 * k + 2] = ez;
@@ -278,17 +265,195 @@ __global__ __launch_bounds__(BT) void k_walk(const InstParams* __restrict__ P,
         tot_open += nopen;
     }
     if (KIND == KIND_FIELD) {
-        // one atomic per wave
         for (int d = 1; d < WAVE; d <<= 1) {
             tot_cell += __shfl_xor(tot_cell, d, WAVE);
             tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
             tot_open += __shfl_xor(tot_open, d, WAVE);
         }
-        if (lane == 0 && (tot_cell | tot_leaf | tot_open)) {
+        if (lane == 0) {
             atomicAdd(&g->n_cell, tot_cell);
             atomicAdd(&g->n_leaf, tot_leaf);
             atomicAdd(&g->n_open, tot_open);
+            atomicAdd(&g->n_pops, tot_pops);
         }
+    }
+}
+
+// ---- float32 compensated walk (field) ---------------------------------
+template <int MODE>
+__global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
+                                                PState st, const NodeRec32* __restrict__ rec,
+                                                WalkIO io)
+{
+    __shared__ uint32_t s_first[NW][STK];
+    __shared__ uint32_t s_cnt[NW][STK];
+    __shared__ uint64_t s_mask[NW][STK];
+    const int wv = threadIdx.x / WAVE;
+    const uint32_t lane = __lane_id();
+    const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
+        tile = rfl(__shfl(tile, 0, WAVE));
+        if ((uint64_t)tile * WAVE >= ntg) break;
+        const uint32_t k = tile * WAVE + lane;
+        uint32_t inst, self;
+        double dtx, dty, dtz;
+        const bool valid = load_target<MODE>(P, st, io, k, ntg, inst, self, dtx, dty, dtz);
+        const InstParams& p = P[inst];
+        const float tx = (float)((dtx - p.R) * p.invR);
+        const float ty = (float)((dty - p.R) * p.invR);
+        const float tz = (float)((dtz - p.R) * p.invR);
+        const float theta2 = (float)p.theta2;
+        const float soft2 = (float)(p.soft2 * p.invR * p.invR);
+        float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        uint32_t ncell = 0, nleaf = 0, nopen = 0;
+
+        uint64_t pending = ballot(valid);
+        while (pending) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
+            const uint64_t grp = rfl64(ballot(valid && inst == gi));
+            pending &= ~grp;
+            uint32_t tf = gi, tc = 1;
+            uint64_t tm = grp;
+            int sp = 0;
+            uint32_t loads = 0;
+            for (;;) {
+                if (tc == 0) {
+                    if (sp == 0) break;
+                    --sp;
+                    tf = rfl(s_first[wv][sp]);
+                    tc = rfl(s_cnt[wv][sp]);
+                    tm = rfl64(s_mask[wv][sp]);
+                }
+                if (++loads > io.n_rec) {
+                    raise_err(g, KERR_DEPTH, tf);
+                    break;
+                }
+                tot_pops++;
+                NodeRec32 mine;
+                if (lane < tc) mine = rec[tf + lane];
+                else mine = NodeRec32{0.f, 0.f, 0.f, 0.f, 0, 0, 0, 0};
+                const bool act = (tm >> lane) & 1ull;
+                uint32_t next_first = 0, next_cnt = 0;
+                uint64_t next_mask = 0;
+                int j = (int)tc - 1;
+                for (; j >= 0; j--) {
+                    const float cx = bcast_f(mine.cx, j), cy = bcast_f(mine.cy, j),
+                                cz = bcast_f(mine.cz, j);
+                    const uint32_t cnt = bcast_u(mine.cnt, j);
+                    const float q = (float)(int)bcast_u((uint32_t)mine.q, j);
+                    bool open = false;
+                    if (act) {
+                        const float dx = tx - cx, dy = ty - cy, dz = tz - cz;
+                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        if (cnt == LEAF) {
+                            if (bcast_u(mine.part, j) != self && d2 != 0.f) {
+                                const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
+                                const float s = q * (inv * inv * inv);
+                                kahan_add(fx, kx, s * dx);
+                                kahan_add(fy, ky, s * dy);
+                                kahan_add(fz, kz, s * dz);
+                                nleaf++;
+                            }
+                        } else if (bcast_f(mine.size2, j) < theta2 * d2) {
+                            if (q != 0.f) {
+                                const float inv = __builtin_amdgcn_rsqf(d2);
+                                const float s = q * (inv * inv * inv);
+                                kahan_add(fx, kx, s * dx);
+                                kahan_add(fy, ky, s * dy);
+                                kahan_add(fz, kz, s * dz);
+                                ncell++;
+                            }
+                        } else {
+                            open = cnt != 0;
+                        }
+                    }
+                    const uint64_t om = rfl64(ballot(open));
+                    if (om) {
+                        if (open) nopen++;
+                        next_first = bcast_u(mine.first, j);
+                        next_cnt = cnt;
+                        next_mask = om;
+                        break;
+                    }
+                }
+                if (j < 0) {
+                    tc = 0;
+                    continue;
+                }
+                if (next_first <= tf + (uint32_t)j || next_first + next_cnt > io.n_rec) {
+                    raise_err(g, KERR_CAPACITY, tf + j);
+                    break;
+                }
+                if (j > 0) {
+                    if (sp >= STK) {
+                        raise_err(g, KERR_DEPTH, tf);
+                        break;
+                    }
+                    s_first[wv][sp] = tf;
+                    s_cnt[wv][sp] = (uint32_t)j;
+                    s_mask[wv][sp] = tm;
+                    sp++;
+                }
+                tf = next_first;
+                tc = next_cnt;
+                tm = next_mask;
+            }
+        }
+        if (valid) {
+            const double K = QE / (K4PE0 * p.R * p.R);
+            io.out[3 * k] += K * (double)fx;
+            io.out[3 * k + 1] += K * (double)fy;
+            io.out[3 * k + 2] += K * (double)fz;
+            if (io.counts) {
+                io.counts[3 * k] = ncell;
+                io.counts[3 * k + 1] = nleaf;
+                io.counts[3 * k + 2] = nopen;
+            }
+        }
+        tot_cell += ncell;
+        tot_leaf += nleaf;
+        tot_open += nopen;
+    }
+    for (int d = 1; d < WAVE; d <<= 1) {
+        tot_cell += __shfl_xor(tot_cell, d, WAVE);
+        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+        tot_open += __shfl_xor(tot_open, d, WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd(&g->n_cell, tot_cell);
+        atomicAdd(&g->n_leaf, tot_leaf);
+        atomicAdd(&g->n_open, tot_open);
+        atomicAdd(&g->n_pops, tot_pops);
+    }
+}
+
+// initial accumulator of every target: the spheroid grain's analytic field
+// (SPH:1815-1818 seeds E with it) or, for the potential, its value at the
+// absolute coordinates as phi_at uses them (SPH:1924-1953); zero for the
+// spherical grain, which is a leaf of the tree
+template <int MODE, int KIND>
+__global__ void k_grain_init(const InstParams* __restrict__ P, const InstState* __restrict__ S,
+                             PState st, WalkIO io)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    uint32_t inst, self;
+    double tx, ty, tz;
+    if (!load_target<MODE>(P, st, io, k, ntg, inst, self, tx, ty, tz)) return;
+    const InstParams& p = P[inst];
+    if (KIND == KIND_FIELD) {
+        double e[3] = {0.0, 0.0, 0.0};
+        if (p.geometry != 0) grain_field(p, S[inst].grain_q, tx, ty, tz, e);
+        io.out[3 * k] = e[0];
+        io.out[3 * k + 1] = e[1];
+        io.out[3 * k + 2] = e[2];
+    } else {
+        io.out[k] = p.geometry != 0 ? grain_phi_abs(p, S[inst].grain_q, tx, ty, tz) : 0.0;
     }
 }
 
@@ -318,12 +483,12 @@ void launch_walk(Bufs& b, hipStream_t s)
     io.out = b.E;
     io.tile_ctr = &b.g->tile_ctr;
     io.n_rec = b.n_rec;
+    const uint32_t nt = b.n_tree ? b.n_tree : 1;
+    k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
     if (b.precision)
-        k_walk<32, MODE_STEP, KIND_FIELD><<<walk_grid(), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.rec64,
-                                                                     b.rec32, io);
+        k_walk32<MODE_STEP><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io);
     else
-        k_walk<64, MODE_STEP, KIND_FIELD><<<walk_grid(), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.rec64,
-                                                                     b.rec32, io);
+        k_walk64<MODE_STEP, KIND_FIELD><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
 }
 
 void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
@@ -338,30 +503,30 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     io.tile_ctr = &b.g->tile_ctr2;
     io.n_rec = b.n_rec;
     k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr2);
-    const int blocks = (int)((n + BT - 1) / BT) < walk_grid() ? (int)((n + BT - 1) / BT) : walk_grid();
-    if (kind == 1)
-        k_walk<64, MODE_POINTS, KIND_PHI><<<blocks ? blocks : 1, BT, 0, s>>>(b.P, b.S, b.g, b.st,
-                                                                          b.rec64, b.rec32, io);
-    else if (b.precision)
-        k_walk<32, MODE_POINTS, KIND_FIELD><<<blocks ? blocks : 1, BT, 0, s>>>(b.P, b.S, b.g, b.st,
-                                                                            b.rec64, b.rec32, io);
-    else
-        k_walk<64, MODE_POINTS, KIND_FIELD><<<blocks ? blocks : 1, BT, 0, s>>>(b.P, b.S, b.g, b.st,
-                                                                            b.rec64, b.rec32, io);
+    const uint32_t nb = (n + BT - 1) / BT ? (n + BT - 1) / BT : 1;
+    const int blocks = (int)nb < walk_grid() ? (int)nb : walk_grid();
+    if (kind == 1) {
+        k_grain_init<MODE_POINTS, KIND_PHI><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
+        k_walk64<MODE_POINTS, KIND_PHI><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    } else {
+        k_grain_init<MODE_POINTS, KIND_FIELD><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
+        if (b.precision)
+            k_walk32<MODE_POINTS><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io);
+        else
+            k_walk64<MODE_POINTS, KIND_FIELD><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    }
 }
 
 // chi_b every 20 steps (pot.c:2409-2418): the potential at one boundary
-// point per instance, before any particle moves
-__global__ __launch_bounds__(BT) void k_chi_b(const InstParams* __restrict__ P, InstState* S,
-                                              Globals* g, PState st,
-                                              const NodeRec64* __restrict__ r64, uint32_t n_inst,
-                                              uint32_t n_rec)
+// point per instance, before any particle moves; one lane walks (a handful
+// of nodes at the domain edge)
+__global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, InstState* S,
+                                              Globals* g, const NodeRec64* __restrict__ r64,
+                                              uint32_t n_inst, uint32_t n_rec)
 {
     if ((g->t % 20) != 0) return;
     const uint32_t inst = blockIdx.x;
-    if (inst >= n_inst) return;
-    // one lane walks; the potential walk is a handful of nodes
-    if (threadIdx.x != 0) return;
+    if (inst >= n_inst || threadIdx.x != 0) return;
     const InstParams& p = P[inst];
     const double x = p.chi_pt[0], y = p.chi_pt[1], z = p.chi_pt[2];
     const double theta2 = p.theta2, soft2 = p.soft2;
@@ -384,7 +549,7 @@ __global__ __launch_bounds__(BT) void k_chi_b(const InstParams* __restrict__ P, 
         } else if (r.size2 < theta2 * d2) {
             if (r.q) ph += (QE * r.q) / (K4PE0 * sqrt(d2));
         } else {
-            if (sp + (int)r.cnt > 8 * STK || r.first <= w || r.first + r.cnt > n_rec) {
+            if (sp + (int)r.cnt > 8 * STK || (r.cnt && r.first <= w) || r.first + r.cnt > n_rec) {
                 raise_err(g, KERR_DEPTH, w);
                 return;
             }
@@ -396,7 +561,7 @@ __global__ __launch_bounds__(BT) void k_chi_b(const InstParams* __restrict__ P, 
 
 void launch_chi_b(Bufs& b, hipStream_t s)
 {
-    k_chi_b<<<b.n_inst, 64, 0, s>>>(b.P, b.S, b.g, b.st, b.rec64, b.n_inst, b.n_rec);
+    k_chi_b<<<b.n_inst, 64, 0, s>>>(b.P, b.S, b.g, b.rec64, b.n_inst, b.n_rec);
 }
 
 }  // namespace potg

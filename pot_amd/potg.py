@@ -52,7 +52,8 @@ class StepOut(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("n_targets", C.c_uint64), ("n_cell", C.c_uint64),
                 ("n_leaf", C.c_uint64), ("n_open", C.c_uint64), ("n_nodes", C.c_uint64),
-                ("n_particles", C.c_uint64), ("max_depth", C.c_uint64)]
+                ("n_particles", C.c_uint64), ("max_depth", C.c_uint64),
+                ("n_range_loads", C.c_uint64)]
 
 
 class PotgError(RuntimeError):

@@ -30,7 +30,7 @@ class Config(C.Structure):
                 ("theta", C.c_double), ("B", C.c_double * 3), ("max_lin_dev", C.c_double),
                 ("mime", C.c_double), ("grain_enabled", C.c_int32), ("sceptic", C.c_int32),
                 ("inplace", C.c_int32), ("rng_kind", C.c_int32), ("philox_seed", C.c_uint64),
-                ("record_macro", C.c_int32), ("pad0", C.c_int32)]
+                ("record_macro", C.c_int32), ("dipoles", C.c_int32)]
 
 
 class WalkStats(C.Structure):
@@ -117,7 +117,8 @@ def dptr(a):
 
 
 def make_config(args, geometry=SPHERE, theta=0.5, B=(0.0, 0.0, 0.0), inplace=1,
-                rng_kind=0, philox_seed=0, record_macro=1, grain_enabled=1, sceptic=1):
+                rng_kind=0, philox_seed=0, record_macro=1, grain_enabled=1, sceptic=1,
+                dipoles=0):
     """Config from reference-style CLI values (strings), parsed as pot.c does."""
     L = lib()
     cfg = Config()
@@ -144,6 +145,7 @@ def make_config(args, geometry=SPHERE, theta=0.5, B=(0.0, 0.0, 0.0), inplace=1,
     cfg.rng_kind = rng_kind
     cfg.philox_seed = philox_seed
     cfg.record_macro = record_macro
+    cfg.dipoles = dipoles
     return cfg
 
 

@@ -179,3 +179,27 @@ def test_batched_instances_equal_single_runs():
         for f in ("x", "y", "z", "vx", "vy", "vz"):
             assert np.array_equal(pb[f], ps[f]), (k, f)
         assert gb["q"] == gs["q"]
+
+
+def test_upload_order_and_reupload_keep_instances_apart():
+    """Instances uploaded in any order, one of them re-uploaded after
+    stepping: each instance's particles stay its own (the upload returns the
+    buffers to reference layout before writing)."""
+    from gpu_common import gpu_config
+    cases = [big_case(1001, geometry=1, A=2.0, seed=51), big_case(1501, seed=52),
+             big_case(801, seed=53)]
+    states = [oracle_sim(c).particles() for c in cases]
+    ctx = G.Context([gpu_config(c, seed=60 + k) for k, c in enumerate(cases)])
+    for k in (2, 0, 1):
+        ctx.upload(k, states[k])
+    ctx.step()
+    ctx.upload(1, states[1])
+    for k in range(3):
+        p, _ = ctx.download(k)
+        assert len(p) == len(states[k])
+        if k == 1:
+            for f in ("x", "y", "z", "vx", "vy", "vz", "species"):
+                assert np.array_equal(p[f], states[1][f]), f
+        else:
+            assert np.array_equal(p["species"], states[k]["species"])
+            assert not np.array_equal(p["x"], states[k]["x"])  # stepped once
