@@ -135,9 +135,13 @@ int potg_download(potg_ctx* ctx, uint32_t inst, potg_particle* parts, potg_grain
 
 /* Advance every instance by one time step (run(), pot.c:4261-4388, with
    move_particles, pot.c:2330-2934).  When compute_epe is nonzero the
-   potential-energy walks of pot.c:2771-2866 run too.  out, if non-NULL,
-   receives n_inst records. */
+   potential-energy walks of pot.c:2771-2866 run too.  With out non-NULL
+   the call waits for the step, reports kernel errors and fills n_inst
+   records; with out NULL it only enqueues the step (errors surface at
+   potg_sync). */
 int potg_step(potg_ctx* ctx, int32_t compute_epe, potg_step_out* out);
+/* Wait for all enqueued work and report any error a kernel raised. */
+int potg_sync(potg_ctx* ctx);
 /* Run `steps` steps as replays of one captured HIP graph (no host
    synchronisation between steps; no per-step outputs). */
 int potg_run(potg_ctx* ctx, uint64_t steps);
@@ -168,9 +172,10 @@ int potg_direct_sum(potg_ctx* ctx, uint32_t inst, const uint32_t* idx, size_t n,
 int potg_stats(potg_ctx* ctx, potg_stats_t* out);
 int potg_stats_reset(potg_ctx* ctx);
 
-/* Wall time (ms) of the last potg_step / potg_run per pipeline stage,
-   measured with HIP events on the context's stream: [0] keys+sort,
-   [1] tree build, [2] walk, [3] push, [4] whole step. */
+/* Per-stage wall time (ms) averaged over the potg_step calls since timing
+   was last enabled, from HIP events on the context's stream: [0] keys +
+   sort, [1] tree build, [2] walk, [3] push + reinjection, [4] whole step.
+   enable turns recording on or off for the steps that follow. */
 int potg_stage_times(potg_ctx* ctx, int32_t enable, double* ms5);
 
 #ifdef __cplusplus

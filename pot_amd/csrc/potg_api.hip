@@ -47,6 +47,8 @@ struct potg_ctx {
     hipEvent_t ev[6];
     int timing;
     double stage_ms[5];
+    std::vector<hipEvent_t> tev;  // per-step stage events, a ring of TRING steps
+    uint32_t t_next, t_count;
     hipGraphExec_t graph[2];
     int graph_ready;
     uint32_t parity;  // which of the two captured steps comes next
@@ -181,9 +183,11 @@ int ensure_staging(potg_ctx* c, size_t bytes)
     return POTG_OK;
 }
 
+constexpr uint32_t TRING = 512;
+
 void record(potg_ctx* c, int k)
 {
-    if (c->timing) (void)hipEventRecord(c->ev[k], c->stream);
+    if (c->timing) (void)hipEventRecord(c->tev[(c->t_next % TRING) * 5 + k], c->stream);
 }
 
 void enqueue_step(potg_ctx* c)
@@ -197,25 +201,38 @@ void enqueue_step(potg_ctx* c)
     launch_tree(b, s);
     launch_targets(b, s);
     launch_chi_b(b, s);
+    launch_walk_init(b, s);
     record(c, 2);
     launch_walk(b, s);
     record(c, 3);
     launch_push(b, s);
     launch_finish_step(b, s);
     record(c, 4);
+    if (c->timing) {
+        c->t_next++;
+        if (c->t_count < TRING) c->t_count++;
+    }
 }
 
+// average stage times over the steps recorded since timing was enabled
 int read_stage_times(potg_ctx* c)
 {
-    if (!c->timing) return POTG_OK;
-    HIPCHK(hipEventSynchronize(c->ev[4]));
-    float ms;
-    for (int k = 0; k < 4; k++) {
-        HIPCHK(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
-        c->stage_ms[k] = ms;
+    if (!c->t_count) return POTG_OK;
+    const uint32_t last = (c->t_next + TRING - 1) % TRING;
+    HIPCHK(hipEventSynchronize(c->tev[last * 5 + 4]));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t n = 0; n < c->t_count; n++) {
+        const uint32_t r = (c->t_next + TRING - 1 - n) % TRING;
+        float ms;
+        for (int k = 0; k < 4; k++) {
+            HIPCHK(hipEventElapsedTime(&ms, c->tev[r * 5 + k], c->tev[r * 5 + k + 1]));
+            acc[k] += ms;
+        }
+        HIPCHK(hipEventElapsedTime(&ms, c->tev[r * 5], c->tev[r * 5 + 4]));
+        acc[4] += ms;
     }
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[4]));
-    c->stage_ms[4] = ms;
+    for (int k = 0; k < 5; k++) c->stage_ms[k] = acc[k] / c->t_count;
+    c->t_count = 0;
     return POTG_OK;
 }
 
@@ -272,6 +289,9 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
         return POTG_ERR_HIP;
     }
     for (int k = 0; k < 6; k++) (void)hipEventCreate(&c->ev[k]);
+    c->tev.resize(TRING * 5);
+    for (auto& e : c->tev) (void)hipEventCreate(&e);
+    c->t_next = c->t_count = 0;
     c->cfg.assign(configs, configs + n_inst);
     Bufs& b = c->b;
     memset(&b, 0, sizeof(b));
@@ -326,6 +346,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.rec64, b.n_rec);
     if (precision == POTG_MIXED) rc |= alloc_n(c, &b.rec32, b.n_rec);
     rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
+    rc |= alloc_n(c, &b.queue, nt);
     rc |= alloc_n(c, &b.E, 3 * nt) | alloc_n(c, &b.inv, n);
     b.q_cap = (uint32_t)(n < 65536 ? 65536 : n);
     rc |= alloc_n(c, &b.q_pts, 3 * (size_t)b.q_cap) | alloc_n(c, &b.q_inst, b.q_cap);
@@ -378,6 +399,8 @@ int potg_destroy(potg_ctx* c)
     if (c->d_aos) (void)hipFree(c->d_aos);
     for (int k = 0; k < 6; k++)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    for (auto& e : c->tev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return POTG_OK;
@@ -473,24 +496,28 @@ int potg_step(potg_ctx* c, int32_t compute_epe, potg_step_out* out)
     enqueue_step(c);
     c->parity ^= 1u;
     HIPCHK(hipGetLastError());
+    if (!out) return POTG_OK;  // asynchronous: errors surface at potg_sync
     int rc = kernel_error(c);
     if (rc) return rc;
-    rc = read_stage_times(c);
-    if (rc) return rc;
-    if (out) {
-        std::vector<InstState> s(c->b.n_inst);
-        HIPCHK(hipMemcpy(s.data(), c->b.S, sizeof(InstState) * c->b.n_inst, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < c->b.n_inst; i++) {
-            memset(&out[i], 0, sizeof(potg_step_out));
-            out[i].abs_q = s[i].out_q;
-            out[i].abs_e = s[i].out_e;
-            out[i].abs_i = s[i].out_i;
-            out[i].n_pushed = s[i].out_pushed;
-            out[i].n_reinj_abs = s[i].out_reinj_abs;
-            out[i].n_reinj_out = s[i].out_reinj_out;
-        }
+    std::vector<InstState> s(c->b.n_inst);
+    HIPCHK(hipMemcpy(s.data(), c->b.S, sizeof(InstState) * c->b.n_inst, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < c->b.n_inst; i++) {
+        memset(&out[i], 0, sizeof(potg_step_out));
+        out[i].abs_q = s[i].out_q;
+        out[i].abs_e = s[i].out_e;
+        out[i].abs_i = s[i].out_i;
+        out[i].n_pushed = s[i].out_pushed;
+        out[i].n_reinj_abs = s[i].out_reinj_abs;
+        out[i].n_reinj_out = s[i].out_reinj_out;
     }
     return POTG_OK;
+}
+
+int potg_sync(potg_ctx* c)
+{
+    if (!c) return set_err("potg_sync: null context"), POTG_ERR_ARG;
+    HIPCHK(hipGetLastError());
+    return kernel_error(c);
 }
 
 int potg_run(potg_ctx* c, uint64_t steps)
@@ -684,6 +711,8 @@ int potg_stats_reset(potg_ctx* c)
 int potg_stage_times(potg_ctx* c, int32_t enable, double* ms5)
 {
     if (!c) return set_err("potg_stage_times: null context"), POTG_ERR_ARG;
+    int rc = read_stage_times(c);
+    if (rc) return rc;
     c->timing = enable ? 1 : 0;
     if (ms5)
         for (int k = 0; k < 5; k++) ms5[k] = c->stage_ms[k];

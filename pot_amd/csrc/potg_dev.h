@@ -92,6 +92,8 @@ struct Globals {
     uint32_t n_children;   // sum of child counts
     uint32_t tile_ctr;     // persistent-walk work counter
     uint32_t tile_ctr2;
+    uint32_t n_reinj;      // reinjection queue length of the current step
+    uint32_t pad_;
     int32_t err;
     uint32_t err_idx;
     uint32_t max_depth;
@@ -110,7 +112,7 @@ struct __align__(16) NodeRec64 {
 };
 struct __align__(16) NodeRec32 {
     float cx, cy, cz, size2;   // relative to the domain centre, in units of R
-    int32_t q;
+    float q;                   // net charge (exact: |q| < 2^24)
     uint32_t first;
     uint32_t cnt;
     uint32_t part;

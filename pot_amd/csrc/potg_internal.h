@@ -55,6 +55,7 @@ struct Bufs {
     uint32_t* flags;
     uint32_t* tpos;
     uint32_t* targets;
+    uint32_t* queue;   // reinjection queue
     double* E;
     uint32_t* inv;     // global reference slot -> sorted slot
     // point queries
@@ -72,6 +73,7 @@ void launch_keys_sort_permute(Bufs& b, hipStream_t s);
 void launch_tree(Bufs& b, hipStream_t s);
 void launch_targets(Bufs& b, hipStream_t s);
 void launch_chi_b(Bufs& b, hipStream_t s);
+void launch_walk_init(Bufs& b, hipStream_t s);  // grain term of each target
 void launch_walk(Bufs& b, hipStream_t s);
 void launch_push(Bufs& b, hipStream_t s);
 void launch_finish_step(Bufs& b, hipStream_t s);

@@ -315,7 +315,8 @@ __device__ bool crosses(const InstParams& p, double px, double py, double pz, co
 __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, InstState* S,
                                               Globals* g, PState st,
                                               const uint32_t* __restrict__ targets,
-                                              const double* __restrict__ E)
+                                              const double* __restrict__ E,
+                                              uint32_t* __restrict__ queue, uint32_t qcap)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ntg = g->n_targets;
@@ -347,9 +348,10 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
         vz += dts * az;
         double x = xh + hdt * vx, y = yh + hdt * vy, z = zh + hdt * vz;
 
-        const uint32_t ref_local = st.ref[i] - p.offset;
-        Stream rs(p.seed, ref_local, t);
-        const double chi_b = S[inst].chi_b;
+        // grain collection (pot.c:2641-2685) and the boundary test
+        // (pot.c:2071-2078); the rare reinjections go to a queue so that the
+        // long SCEPTIC sampling runs in lanes of its own (k_reinject)
+        int fate = 0;  // 1: absorbed, 2: left the domain
         if (p.grain_enabled && ((double)t * p.dt) > p.settle) {
             const double d[3] = {x - x0, y - y0, z - z0};
             if (crosses(p, x, y, z, d)) {
@@ -360,18 +362,20 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
                 atomicAdd(&S[inst].acc_p[1], p.mass[s] * vy);
                 atomicAdd(&S[inst].acc_p[2], p.mass[s] * vz);
                 atomicAdd(&S[inst].n_reinj_abs, 1u);
-                if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
-                else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
+                fate = 1;
             }
         }
-        {
+        if (!fate) {
             const double rad = p.R;
             const double dx = x - rad, dy = y - rad, dz = z - rad;
             if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
                 atomicAdd(&S[inst].n_reinj_out, 1u);
-                if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
-                else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
+                fate = 2;
             }
+        }
+        if (fate) {
+            const uint32_t q = atomicAdd(&g->n_reinj, 1u);
+            if (q < qcap) queue[q] = (i << 1) | (fate == 1 ? 1u : 0u);
         }
         st.x[i] = x;
         st.y[i] = y;
@@ -388,6 +392,46 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
         const uint64_t grp = ballot(valid && inst == gi);
         pending &= ~grp;
         if ((int)__lane_id() == lead) atomicAdd(&S[gi].n_pushed, (uint32_t)__popcll(grp));
+    }
+}
+
+// Reinjections queued by k_push: (slot << 1) | absorbed.  An absorbed
+// particle is reinjected and then meets the boundary test like any other
+// (pot.c:2676, 2693); one that left the domain is reinjected once.  The
+// draws come from the particle's (seed, index, step) stream, so the order
+// of the queue does not matter.
+__global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ P, InstState* S,
+                                                  Globals* g, PState st,
+                                                  const uint32_t* __restrict__ queue, uint32_t qcap)
+{
+    const uint32_t n = g->n_reinj < qcap ? g->n_reinj : qcap;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const uint32_t code = queue[e];
+        const uint32_t i = code >> 1;
+        const uint32_t inst = st.inst[i];
+        const InstParams& p = P[inst];
+        const int s = st.spec[i];
+        Stream rs(p.seed, st.ref[i] - p.offset, g->t);
+        const double chi_b = S[inst].chi_b;
+        double x = st.x[i], y = st.y[i], z = st.z[i];
+        double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
+        if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
+        else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
+        if (code & 1u) {
+            const double rad = p.R;
+            const double dx = x - rad, dy = y - rad, dz = z - rad;
+            if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
+                atomicAdd(&S[inst].n_reinj_out, 1u);
+                if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
+                else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
+            }
+        }
+        st.x[i] = x;
+        st.y[i] = y;
+        st.z[i] = z;
+        st.vx[i] = vx;
+        st.vy[i] = vy;
+        st.vz[i] = vz;
     }
 }
 
@@ -587,7 +631,8 @@ void launch_targets(Bufs& b, hipStream_t s)
 
 void launch_push(Bufs& b, hipStream_t s)
 {
-    k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E);
+    k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.queue, b.n_tree);
+    k_reinject<<<64, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.n_tree);
 }
 
 void launch_finish_step(Bufs& b, hipStream_t s)

@@ -93,6 +93,7 @@ __global__ void k_begin_step(InstParams* __restrict__ P, InstState* __restrict__
         g->t = t;
         g->tile_ctr = 0;
         g->tile_ctr2 = 0;
+        g->n_reinj = 0;
     }
 }
 
@@ -390,7 +391,7 @@ __global__ void k_empty_roots(const InstParams* __restrict__ P, uint32_t n_inst,
         NodeRec32 f;
         f.cx = f.cy = f.cz = 0.f;
         f.size2 = 0.f;
-        f.q = 0;
+        f.q = 0.f;
         f.first = 0;
         f.cnt = 0;
         f.part = 0;
@@ -406,7 +407,7 @@ __device__ inline void put32(NodeRec32* r32, uint32_t w, const InstParams& p, do
     f.cy = (float)((cy - p.R) * p.invR);
     f.cz = (float)((cz - p.R) * p.invR);
     f.size2 = (float)(s2 * p.invR * p.invR);
-    f.q = q;
+    f.q = (float)q;
     f.first = first;
     f.cnt = cnt;
     f.part = part;

@@ -280,6 +280,10 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
 }
 
 // ---- float32 compensated walk (field) ---------------------------------
+// The current child range is staged in LDS (one record per lane, written by
+// lanes 0..cnt-1) and each child is read back by every lane with two
+// broadcast ds_read_b128; the interactions of one range are summed in plain
+// float32 and the range total enters the per-particle Kahan sum.
 template <int MODE>
 __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec32* __restrict__ rec,
@@ -288,6 +292,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
     __shared__ uint32_t s_first[NW][STK];
     __shared__ uint32_t s_cnt[NW][STK];
     __shared__ uint64_t s_mask[NW][STK];
+    __shared__ float4 s_rec[NW][8][2];
     const int wv = threadIdx.x / WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
@@ -334,38 +339,43 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     break;
                 }
                 tot_pops++;
-                NodeRec32 mine;
-                if (lane < tc) mine = rec[tf + lane];
-                else mine = NodeRec32{0.f, 0.f, 0.f, 0.f, 0, 0, 0, 0};
+                if (lane < tc) {
+                    const float4* src = reinterpret_cast<const float4*>(rec + tf + lane);
+                    s_rec[wv][lane][0] = src[0];
+                    s_rec[wv][lane][1] = src[1];
+                }
+                // the wave reads what its own lanes just wrote: LDS keeps a
+                // wave's accesses in order; keep the compiler from moving them
+                __builtin_amdgcn_wave_barrier();
                 const bool act = (tm >> lane) & 1ull;
+                float px = 0.f, py = 0.f, pz = 0.f;  // this range's float32 partial
                 uint32_t next_first = 0, next_cnt = 0;
                 uint64_t next_mask = 0;
                 int j = (int)tc - 1;
                 for (; j >= 0; j--) {
-                    const float cx = bcast_f(mine.cx, j), cy = bcast_f(mine.cy, j),
-                                cz = bcast_f(mine.cz, j);
-                    const uint32_t cnt = bcast_u(mine.cnt, j);
-                    const float q = (float)(int)bcast_u((uint32_t)mine.q, j);
+                    const float4 a = s_rec[wv][j][0];  // cx, cy, cz, size2
+                    const float4 b = s_rec[wv][j][1];  // q, first, cnt, part
+                    const uint32_t cnt = rfl(__float_as_uint(b.z));
                     bool open = false;
                     if (act) {
-                        const float dx = tx - cx, dy = ty - cy, dz = tz - cz;
+                        const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
                         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         if (cnt == LEAF) {
-                            if (bcast_u(mine.part, j) != self && d2 != 0.f) {
+                            if (__float_as_uint(b.w) != self && d2 != 0.f) {
                                 const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
-                                const float s = q * (inv * inv * inv);
-                                kahan_add(fx, kx, s * dx);
-                                kahan_add(fy, ky, s * dy);
-                                kahan_add(fz, kz, s * dz);
+                                const float s = b.x * (inv * inv * inv);
+                                px = __builtin_fmaf(s, dx, px);
+                                py = __builtin_fmaf(s, dy, py);
+                                pz = __builtin_fmaf(s, dz, pz);
                                 nleaf++;
                             }
-                        } else if (bcast_f(mine.size2, j) < theta2 * d2) {
-                            if (q != 0.f) {
+                        } else if (a.w < theta2 * d2) {
+                            if (b.x != 0.f) {
                                 const float inv = __builtin_amdgcn_rsqf(d2);
-                                const float s = q * (inv * inv * inv);
-                                kahan_add(fx, kx, s * dx);
-                                kahan_add(fy, ky, s * dy);
-                                kahan_add(fz, kz, s * dz);
+                                const float s = b.x * (inv * inv * inv);
+                                px = __builtin_fmaf(s, dx, px);
+                                py = __builtin_fmaf(s, dy, py);
+                                pz = __builtin_fmaf(s, dz, pz);
                                 ncell++;
                             }
                         } else {
@@ -375,12 +385,15 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     const uint64_t om = rfl64(ballot(open));
                     if (om) {
                         if (open) nopen++;
-                        next_first = bcast_u(mine.first, j);
+                        next_first = rfl(__float_as_uint(b.y));
                         next_cnt = cnt;
                         next_mask = om;
                         break;
                     }
                 }
+                kahan_add(fx, kx, px);
+                kahan_add(fy, ky, py);
+                kahan_add(fz, kz, pz);
                 if (j < 0) {
                     tc = 0;
                     continue;
@@ -475,7 +488,7 @@ int walk_grid()
 
 }  // namespace
 
-void launch_walk(Bufs& b, hipStream_t s)
+static WalkIO step_io(Bufs& b)
 {
     WalkIO io{};
     io.targets = b.targets;
@@ -483,8 +496,19 @@ void launch_walk(Bufs& b, hipStream_t s)
     io.out = b.E;
     io.tile_ctr = &b.g->tile_ctr;
     io.n_rec = b.n_rec;
+    return io;
+}
+
+void launch_walk_init(Bufs& b, hipStream_t s)
+{
+    WalkIO io = step_io(b);
     const uint32_t nt = b.n_tree ? b.n_tree : 1;
     k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
+}
+
+void launch_walk(Bufs& b, hipStream_t s)
+{
+    WalkIO io = step_io(b);
     if (b.precision)
         k_walk32<MODE_STEP><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io);
     else

@@ -26,7 +26,7 @@ ERRORS = {-1: "POTG_ERR_ARG", -2: "POTG_ERR_HIP", -3: "POTG_ERR_OUT_OF_BOX",
 EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "potg_download",
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
-           "potg_stage_times"]
+           "potg_stage_times", "potg_sync"]
 
 
 class Config(C.Structure):
@@ -90,6 +90,7 @@ def lib():
     L.potg_stats.argtypes = [P, C.POINTER(Stats)]
     L.potg_stats_reset.argtypes = [P]
     L.potg_stage_times.argtypes = [P, C.c_int32, dp]
+    L.potg_sync.argtypes = [P]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
@@ -161,6 +162,12 @@ class Context:
         _check(self.L.potg_download(self.h, inst, out.ctypes.data, C.byref(g)))
         return out, {"q": g.q, "m": g.m, "v": tuple(g.v), "chi_b": g.chi_b,
                      "tot_e": g.tot_absor_e}
+
+    def step_async(self, compute_epe=0):
+        _check(self.L.potg_step(self.h, compute_epe, None))
+
+    def sync(self):
+        _check(self.L.potg_sync(self.h))
 
     def step(self, compute_epe=0):
         out = (StepOut * self.n_inst)()
