@@ -61,7 +61,7 @@ def build_lib(jobs=8):
     return LIB
 
 
-HOST_LIB_SOURCES = ["pot_init.c"]
+HOST_LIB_SOURCES = ["pot_init.c", "pot_macro.c", "pot_state.c", "pot_info.c"]
 HOST_MAIN = "pot_main.c"
 HOST_LIB = os.path.join(BUILD, "libpothost.so")
 
