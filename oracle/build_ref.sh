@@ -61,17 +61,17 @@ build_sph() {  # name shape theta bx by bz
             -DRANDOMNESS_SOURCE='"seed.bin"' -o "$OUT/$name" -lm
 }
 
-# spherical grain, SCEPTIC reinjection, Boris (SURVEY appendix A, configs 1, 2, 5)
+# spherical grain, SCEPTIC reinjection, Boris
 build_pot pot_sphere 0.5 0.0 0.0 0.0
 build_pot pot_sphere_bz 0.5 0.0 0.0 50.0
 build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
 build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
-# benchmark workloads (pot_amd/workloads.py): config 2 (B along z, r_ge =
-# lambda_D / 2) and config 3 (prolate, B 30 degrees off the axis)
-build_pot pot_cfg2 0.5 0.0 0.0 3.72363
-# spheroidal grain (pot-spheroid.c; config 3 uses B tilted 30 degrees off the x axis)
+# benchmark workloads (pot_amd/workloads.py): sphere_2m_bz (B along z,
+# r_ge = lambda_D / 2) and prolate_4m_tilted_b (B 30 degrees off the axis)
+build_pot pot_sphere_2m_bz 0.5 0.0 0.0 3.72363
+# spheroidal grain (pot-spheroid.c; prolate_4m_tilted_b has B 30 degrees off the x axis)
 build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
 build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0
-build_sph pot_cfg3 PROLATE 0.5 1.61238 0.0 0.93091
+build_sph pot_prolate_4m PROLATE 0.5 1.61238 0.0 0.93091
 
 echo "build_ref: built $(ls "$OUT" | wc -l) reference variants in $OUT"

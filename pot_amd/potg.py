@@ -161,7 +161,7 @@ class Context:
         g = Grain()
         _check(self.L.potg_download(self.h, inst, out.ctypes.data, C.byref(g)))
         return out, {"q": g.q, "m": g.m, "v": tuple(g.v), "chi_b": g.chi_b,
-                     "tot_e": g.tot_absor_e}
+                     "tot_e": g.tot_absor_e, "epe": tuple(g.epe)}
 
     def step_async(self, compute_epe=0):
         _check(self.L.potg_step(self.h, compute_epe, None))

@@ -53,19 +53,19 @@ def plasma(n_e, r_over_ld, te=11600.0, te_over_ti=1.0, a_over_ld=0.1, rge_over_l
             "lambda_D": ld, "R": R}
 
 
-def config1():
+def sphere_16k():
     """Plain CPU-runnable case: 8192 e + 8192 i, a = 0.1 lambda_D, R = 10 lambda_D,
     no B, no flow."""
     return plasma(8192, 10.0)
 
 
-def config2():
+def sphere_2m_bz():
     """1M e + 1M i, R = 15 lambda_D, a = 0.1 lambda_D, B along z with
     r_ge = 0.5 lambda_D, Te/Ti = 1."""
     return plasma(1000000, 15.0, rge_over_ld=0.5)
 
 
-def config3():
+def prolate_4m_tilted_b():
     """Prolate grain, aspect ratio 2, semi-major 0.2 lambda_D, 2M e + 2M i,
     R = 15 lambda_D, Te/Ti = 10, B tilted 30 degrees off the spheroid axis
     (x in the reference, SPH:1731-1737); r_ge = 0.5 lambda_D."""
@@ -74,7 +74,7 @@ def config3():
                   geometry=1, A=2.0, semi_major=0.2)
 
 
-def config4(n_inst=64, seed=2024):
+def batched_sweep(n_inst=64, seed=2024):
     """The batched sweep: per-instance N log-uniform in [20k, 500k], a in
     [0.05, 0.5] lambda_D, r_ge/lambda_D in [0.2, 5], ion drift Mach in
     [0, 1.5] along B (x in the reference's drift convention), Te/Ti in {1, 10}."""
@@ -90,7 +90,7 @@ def config4(n_inst=64, seed=2024):
     return out
 
 
-def config5(theta=0.5):
+def flowing_16m(theta=0.5):
     """8M e + 8M i, R = 20 lambda_D, Te/Ti = 10, ion drift Mach 1 along B,
     B along x (the reference's drift axis) with r_ge = lambda_D."""
     return plasma(8000000, 20.0, te_over_ti=10.0, rge_over_ld=1.0, b_dir=(1.0, 0.0, 0.0),

@@ -145,6 +145,31 @@ def test_step_matches_oracle_frozen_philox(case):
     assert gr["q"] == sim.state()["q"]
 
 
+@pytest.mark.parametrize("case", SMALL, ids=[c["name"] for c in SMALL])
+def test_step_potential_energies_match_oracle(case):
+    """compute_epe steps: the potential each particle feels at its
+    half-drifted position (phi_particle_feels, pot.c:1934-2018) summed per
+    species, and the grain's with its updated charge (pot.c:2852-2863),
+    against the oracle (whose sums the reference's macro files pin).  The
+    oracle adds in long double in index order, the GPU in float64 in a
+    fixed tree order: 1e-11 relative."""
+    import ctypes as C
+    sim = oracle_sim(case, inplace=0, rng_kind=O.RNG_PHILOX, philox_seed=77)
+    ctx = gpu_ctx(case, sim.particles(), seed=77)
+    for step in range(3):
+        sim.run_step()  # every step up to 200 computes the energies
+        ctx.step(compute_epe=1)
+        epe = (C.c_longdouble * 3)()
+        sim.L.or_get_state(sim.h, None, None, None, None, None, epe)
+        _, gr = ctx.download(0)
+        for s in range(3):
+            want = float(epe[s])
+            if want == 0.0:
+                assert gr["epe"][s] == 0.0, (step, s)
+            else:
+                assert abs(gr["epe"][s] - want) <= 1e-11 * abs(want), (step, s, gr["epe"][s], want)
+
+
 def test_graph_replay_equals_eager_steps():
     case = big_case(20001, seed=21)
     parts = oracle_sim(case).particles()
