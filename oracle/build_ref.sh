@@ -66,6 +66,8 @@ build_pot pot_sphere 0.5 0.0 0.0 0.0
 build_pot pot_sphere_bz 0.5 0.0 0.0 50.0
 build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
 build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
+# dipole moments and the dipole walk (DIPOLES, pot.c:99-105)
+build_pot pot_sphere_dip 0.5 0.0 0.0 20.0 -DDIPOLES
 # benchmark workloads (pot_amd/workloads.py): sphere_2m_bz (B along z,
 # r_ge = lambda_D / 2) and prolate_4m_tilted_b (B 30 degrees off the axis)
 build_pot pot_sphere_2m_bz 0.5 0.0 0.0 3.72363

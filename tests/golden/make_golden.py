@@ -87,6 +87,9 @@ def cases():
     obl["-A"] = "0.5"
     cs.append({"name": "oblate", "bin": "pot_oblate", "geometry": 2, "theta": 0.5,
                "B": [0.0, 0.0, 0.0], "args": obl, "steps": 60, "seed": 6})
+    cs.append({"name": "sphere_dipoles", "bin": "pot_sphere_dip", "geometry": 0, "theta": 0.5,
+               "B": [0.0, 0.0, 20.0], "args": dict(base), "steps": 60, "seed": 7,
+               "dipoles": 1})
     return cs
 
 

@@ -28,7 +28,7 @@ def big_case(N, geometry=0, A=1.0, te=11600.0, ti=11600.0, B=(0.0, 0.0, 0.0), se
 def oracle_sim(case, inplace=0, rng_kind=O.RNG_WELL512, philox_seed=0):
     cfg = O.make_config(case["args"], geometry=case["geometry"], theta=case["theta"],
                         B=tuple(case["B"]), inplace=inplace, rng_kind=rng_kind,
-                        philox_seed=philox_seed)
+                        philox_seed=philox_seed, dipoles=case.get("dipoles", 0))
     sim = O.Sim(cfg)
     sim.init_particles(splitmix64_words(case["seed"]))
     return sim

@@ -24,7 +24,7 @@ CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 
 def run_oracle(case, steps):
     cfg = O.make_config(case["args"], geometry=case["geometry"], theta=case["theta"],
-                        B=tuple(case["B"]))
+                        B=tuple(case["B"]), dipoles=case.get("dipoles", 0))
     sim = O.Sim(cfg)
     sim.init_particles(splitmix64_words(case["seed"]))
     for _ in range(steps):
