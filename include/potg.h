@@ -80,6 +80,8 @@ typedef struct potg_config {
     int32_t grain_enabled; /* absorbing grain (!DISABLE_GRAIN, pot.c:30) */
     int32_t sceptic;       /* SCEPTIC_REINJECTION (pot.c:180-182); must be 1 */
     uint64_t seed;         /* key of the counter-based reinjection stream */
+    int32_t dipoles;       /* DIPOLES (pot.c:99-105): dipole moments and terms */
+    int32_t pad;
 } potg_config;
 
 /* Scalar state of an instance besides its particles. */
@@ -154,6 +156,12 @@ int potg_build_tree(potg_ctx* ctx);
 int potg_tree_canon(potg_ctx* ctx, uint32_t inst, int32_t* depth, int32_t* is_leaf,
                     uint32_t* part, int32_t* q, uint32_t* aq, double* c, double* x1x2,
                     int32_t* nchild, int64_t cap, int64_t* n_out);
+
+/* Dipole moments (x, y, z per node, about the centre of |q|) of instance
+   inst's tree in the order of potg_tree_canon (DIPOLES, pot.c:1290-1307);
+   zero for leaves and for instances without dipoles. */
+int potg_tree_canon_dipoles(potg_ctx* ctx, uint32_t inst, double* p, int64_t cap,
+                            int64_t* n_out);
 
 /* Field at n points of instance inst using the current tree (built by
    potg_build_tree or the last step), as field_particle_feels_at does:

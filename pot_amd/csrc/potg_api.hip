@@ -155,6 +155,7 @@ void fill_params(InstParams& p, const potg_config& k)
     p.grain_enabled = k.grain_enabled;
     p.sceptic = k.sceptic;
     p.seed = k.seed;
+    p.dipoles = k.dipoles != 0;
     p.N = (uint32_t)k.N;
     for (int s = 0; s < 3; s++) p.first_idx[s] = NONE;
 }
@@ -190,7 +191,7 @@ void record(potg_ctx* c, int k)
     if (c->timing) (void)hipEventRecord(c->tev[(c->t_next % TRING) * 5 + k], c->stream);
 }
 
-void enqueue_step(potg_ctx* c)
+void enqueue_step(potg_ctx* c, int compute_epe)
 {
     Bufs& b = c->b;
     hipStream_t s = c->stream;
@@ -204,9 +205,10 @@ void enqueue_step(potg_ctx* c)
     launch_walk_init(b, s);
     record(c, 2);
     launch_walk(b, s);
+    if (compute_epe) launch_epe(b, s);
     record(c, 3);
     launch_push(b, s);
-    launch_finish_step(b, s);
+    launch_finish_step(b, compute_epe, s);
     record(c, 4);
     if (c->timing) {
         c->t_next++;
@@ -303,6 +305,13 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
     b.key_bits = 64;
+    // walk ranges with at most this many targets in the mask are evaluated
+    // as target x child pairs; POTG_WALK_PAIR_MAX overrides it (0..64)
+    b.pair_max = 16;
+    if (const char* e = getenv("POTG_WALK_PAIR_MAX")) {
+        const long v = strtol(e, nullptr, 10);
+        if (v >= 0 && v <= 64) b.pair_max = (uint32_t)v;
+    }
 
     c->hp.resize(n_inst);
     uint32_t off = 0, tb = 0;
@@ -345,9 +354,17 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
     rc |= alloc_n(c, &b.rec64, b.n_rec);
     if (precision == POTG_MIXED) rc |= alloc_n(c, &b.rec32, b.n_rec);
+    bool any_dip = false;
+    for (uint32_t i = 0; i < n_inst; i++) any_dip |= configs[i].dipoles != 0;
+    if (any_dip) {
+        rc |= alloc_n(c, &b.mpx, ci) | alloc_n(c, &b.mpy, ci) | alloc_n(c, &b.mpz, ci);
+        rc |= alloc_n(c, &b.dip64, 3 * (size_t)b.n_rec);
+        if (precision == POTG_MIXED) rc |= alloc_n(c, &b.dip32, b.n_rec);
+    }
     rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
     rc |= alloc_n(c, &b.queue, nt);
     rc |= alloc_n(c, &b.E, 3 * nt) | alloc_n(c, &b.inv, n);
+    rc |= alloc_n(c, &b.phi, n) | alloc_n(c, &b.epe_part, 2 * (size_t)EPE_BLOCKS * n_inst);
     b.q_cap = (uint32_t)(n < 65536 ? 65536 : n);
     rc |= alloc_n(c, &b.q_pts, 3 * (size_t)b.q_cap) | alloc_n(c, &b.q_inst, b.q_cap);
     rc |= alloc_n(c, &b.q_self, b.q_cap) | alloc_n(c, &b.q_out, 3 * (size_t)b.q_cap);
@@ -492,8 +509,7 @@ int potg_step(potg_ctx* c, int32_t compute_epe, potg_step_out* out)
 {
     if (!c) return set_err("potg_step: null context"), POTG_ERR_ARG;
     if (all_uploaded(c)) return POTG_ERR_ARG;
-    (void)compute_epe;
-    enqueue_step(c);
+    enqueue_step(c, compute_epe != 0);
     c->parity ^= 1u;
     HIPCHK(hipGetLastError());
     if (!out) return POTG_OK;  // asynchronous: errors surface at potg_sync
@@ -531,7 +547,7 @@ int potg_run(potg_ctx* c, uint64_t steps)
         for (int k = 0; k < 2; k++) {
             hipGraph_t gr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c);
+            enqueue_step(c, 0);
             HIPCHK(hipStreamEndCapture(c->stream, &gr));
             HIPCHK(hipGraphInstantiate(&c->graph[k], gr, nullptr, nullptr, 0));
             HIPCHK(hipGraphDestroy(gr));
@@ -563,50 +579,84 @@ int potg_build_tree(potg_ctx* c)
     return kernel_error(c);
 }
 
-int potg_tree_canon(potg_ctx* c, uint32_t inst, int32_t* depth, int32_t* is_leaf, uint32_t* part,
-                    int32_t* q, uint32_t* aq, double* cc, double* size2, int32_t* nchild,
-                    int64_t cap, int64_t* n_out)
+}  // extern "C"
+
+namespace {
+// Visit instance inst's tree in canonical order (depth-first pre-order,
+// children in the reference order): visit(n, walk id, record, depth).
+template <class F>
+int canon_walk(potg_ctx* c, uint32_t inst, int64_t cap, int64_t* n_out, const char* who, F visit)
 {
-    if (!c || inst >= c->b.n_inst || !n_out) return set_err("potg_tree_canon: bad arguments"), POTG_ERR_ARG;
+    if (!c || inst >= c->b.n_inst || !n_out)
+        return set_err("%s: bad arguments", who), POTG_ERR_ARG;
     Globals g;
     HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
     const size_t nrec = c->b.n_inst + g.n_children;
+    if (nrec > c->b.n_rec) return set_err("%s: no tree", who), POTG_ERR_ARG;
     std::vector<NodeRec64> rec(nrec);
-    std::vector<uint32_t> ref(c->b.n_total);
-    std::vector<int32_t> mq;
     HIPCHK(hipMemcpy(rec.data(), c->b.rec64, nrec * sizeof(NodeRec64), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ref.data(), c->b.st.ref, ref.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    const uint32_t off = c->hp[inst].offset;
     std::vector<std::pair<uint32_t, int>> stk;
     int64_t n = 0;
-    if (c->hp[inst].tree_n == 0) {
-        *n_out = 0;
-        return POTG_OK;
-    }
+    *n_out = 0;
+    if (c->hp[inst].tree_n == 0) return POTG_OK;
     stk.push_back({inst, 0});
     while (!stk.empty()) {
         const uint32_t w = stk.back().first;
         const int d = stk.back().second;
         stk.pop_back();
-        if (n >= cap) return set_err("potg_tree_canon: capacity"), POTG_ERR_ARG;
+        if (n >= cap) return set_err("%s: capacity", who), POTG_ERR_ARG;
+        if (w >= nrec) return set_err("%s: bad record", who), POTG_ERR_ARG;
         const NodeRec64& r = rec[w];
-        const bool leaf = r.cnt == LEAF;
-        depth[n] = d;
-        is_leaf[n] = leaf;
-        part[n] = leaf ? ref[r.part] - off : 0xFFFFFFFFu;
-        q[n] = r.q;
-        aq[n] = leaf ? (uint32_t)abs(r.q) : r.part;
-        cc[3 * n] = r.cx;
-        cc[3 * n + 1] = r.cy;
-        cc[3 * n + 2] = r.cz;
-        size2[n] = leaf ? 0.0 : r.size2;
-        nchild[n] = leaf ? 0 : (int32_t)r.cnt;
+        visit(n, w, r, d);
         n++;
-        if (!leaf)
+        if (r.cnt != LEAF)
             for (int k = (int)r.cnt - 1; k >= 0; k--) stk.push_back({r.first + (uint32_t)k, d + 1});
     }
     *n_out = n;
     return POTG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int potg_tree_canon(potg_ctx* c, uint32_t inst, int32_t* depth, int32_t* is_leaf, uint32_t* part,
+                    int32_t* q, uint32_t* aq, double* cc, double* size2, int32_t* nchild,
+                    int64_t cap, int64_t* n_out)
+{
+    if (!c) return set_err("potg_tree_canon: null context"), POTG_ERR_ARG;
+    std::vector<uint32_t> ref(c->b.n_total);
+    HIPCHK(hipMemcpy(ref.data(), c->b.st.ref, ref.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    const uint32_t off = inst < c->b.n_inst ? c->hp[inst].offset : 0;
+    return canon_walk(c, inst, cap, n_out, "potg_tree_canon",
+                      [&](int64_t n, uint32_t, const NodeRec64& r, int d) {
+                          const bool leaf = r.cnt == LEAF;
+                          depth[n] = d;
+                          is_leaf[n] = leaf;
+                          part[n] = leaf ? ref[r.part] - off : 0xFFFFFFFFu;
+                          q[n] = r.q;
+                          aq[n] = leaf ? (uint32_t)abs(r.q) : r.part;
+                          cc[3 * n] = r.cx;
+                          cc[3 * n + 1] = r.cy;
+                          cc[3 * n + 2] = r.cz;
+                          size2[n] = leaf ? 0.0 : r.size2;
+                          nchild[n] = leaf ? 0 : (int32_t)r.cnt;
+                      });
+}
+
+int potg_tree_canon_dipoles(potg_ctx* c, uint32_t inst, double* p, int64_t cap, int64_t* n_out)
+{
+    if (!c || !p) return set_err("potg_tree_canon_dipoles: bad arguments"), POTG_ERR_ARG;
+    std::vector<double> dip;
+    if (c->b.dip64) {
+        dip.resize(3 * (size_t)c->b.n_rec);
+        HIPCHK(hipMemcpy(dip.data(), c->b.dip64, dip.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    }
+    return canon_walk(c, inst, cap, n_out, "potg_tree_canon_dipoles",
+                      [&](int64_t n, uint32_t w, const NodeRec64&, int) {
+                          for (int k = 0; k < 3; k++)
+                              p[3 * n + k] = dip.empty() ? 0.0 : dip[3 * (size_t)w + k];
+                      });
 }
 
 static int run_points(potg_ctx* c, uint32_t inst, const double* xyz, const uint32_t* exclude,

@@ -24,6 +24,7 @@ constexpr uint32_t LEAF = 0xFFFFFFFFu;   // NodeRec.cnt marker of a leaf
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t LEAF_BIT = 0x80000000u; // child-slot code of a leaf
 constexpr int WAVE = 64;
+constexpr int EPE_BLOCKS = 64;    // partial sums per instance of the PE reduction
 
 // error codes raised by kernels (mirrors POTG_ERR_*)
 constexpr int KERR_OUT_OF_BOX = 1;
@@ -62,7 +63,7 @@ struct InstParams {
     uint32_t tree_begin; // first sorted slot of the instance's tree particles
     uint32_t tree_n;     // tree particles (N, or N - 1 when the grain is analytic)
     uint32_t first_idx[3]; // lowest reference index of each species
-    uint32_t pad;
+    int32_t dipoles;       // DIPOLES: dipole moments in the tree and the walks
 };
 
 // Mutable per-instance state on the device.

@@ -6,6 +6,15 @@
 
 namespace potg {
 
+// Per-internal-node moment arrays of the build (and their record copies).
+struct NodeMoments {
+    double *cx, *cy, *cz, *px, *py, *pz;
+    int32_t* q;
+    uint32_t* aq;
+    double* dip64;
+    float4* dip32;
+};
+
 // Particle state in sorted (octree) order, structure of arrays.
 struct PState {
     double *x, *y, *z, *vx, *vy, *vz;
@@ -21,6 +30,7 @@ struct Bufs {
     int precision;     // POTG_FP64 / POTG_MIXED
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
+    uint32_t pair_max; // walk: ranges with at most this many targets go pair-parallel
     InstParams* P;
     InstState* S;
     Globals* g;
@@ -47,8 +57,11 @@ struct Bufs {
     double *mcx, *mcy, *mcz;
     int32_t* mq;
     uint32_t* maq;
+    double *mpx, *mpy, *mpz;  // dipole moments of internal nodes (DIPOLES only)
     NodeRec64* rec64;
     NodeRec32* rec32;
+    double* dip64;     // dipole of each record, 3 per record (DIPOLES only)
+    float4* dip32;     // the same in units of R for the float32 walk
     uint32_t n_rec;    // record capacity
     uint32_t cap_int;  // internal-node capacity
     // targets of the walk and push
@@ -58,6 +71,9 @@ struct Bufs {
     uint32_t* queue;   // reinjection queue
     double* E;
     uint32_t* inv;     // global reference slot -> sorted slot
+    // potential energies (compute_epe steps)
+    double* phi;       // potential felt by each sorted slot
+    double* epe_part;  // EPE_BLOCKS x {electron, ion} partial sums per instance
     // point queries
     double* q_pts;
     uint32_t* q_inst;
@@ -76,7 +92,8 @@ void launch_chi_b(Bufs& b, hipStream_t s);
 void launch_walk_init(Bufs& b, hipStream_t s);  // grain term of each target
 void launch_walk(Bufs& b, hipStream_t s);
 void launch_push(Bufs& b, hipStream_t s);
-void launch_finish_step(Bufs& b, hipStream_t s);
+// compute_epe nonzero: fold the potential-energy sums of launch_epe
+void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s);
 void launch_epe(Bufs& b, hipStream_t s);
 // upload / download helpers
 void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n, hipStream_t s);

@@ -445,12 +445,24 @@ __global__ void k_target_flags(const InstParams* __restrict__ P, const InstState
     flags[i] = (st.spec[i] != 0 && local >= S[inst].start) ? 1u : 0u;
 }
 
-__global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals* g, uint32_t n_inst)
+__global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals* g, uint32_t n_inst,
+                         int compute_epe, const double* __restrict__ epe_part,
+                         const double* __restrict__ phi, const uint32_t* __restrict__ inv)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n_inst) {
         InstState& s = S[k];
         const InstParams& p = P[k];
+        if (compute_epe) {
+            // electron and ion totals (pot.c:2775-2792), partials in a fixed order
+            double se = 0.0, si = 0.0;
+            for (int b = 0; b < EPE_BLOCKS; b++) {
+                se += epe_part[2 * (k * EPE_BLOCKS + b)];
+                si += epe_part[2 * (k * EPE_BLOCKS + b) + 1];
+            }
+            s.epe[2] = se;
+            s.epe[1] = si;
+        }
         if (p.grain_enabled) {
             // pot.c:2831-2836
             s.grain_m += (double)s.acc_i * p.mass[1] + (double)s.acc_e * p.mass[2];
@@ -459,6 +471,10 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
             s.grain_v[2] += s.acc_p[2] / s.grain_m;
             s.grain_q += s.acc_q;
             s.tot_e += s.acc_e;
+            // the grain's own potential energy with its updated charge
+            // (pot.c:2852-2863)
+            if (compute_epe)
+                s.epe[0] = s.grain_q ? ((double)s.grain_q * QE) * phi[inv[p.offset]] : 0.0;
         }
         s.out_q = s.acc_q;
         s.out_e = s.acc_e;
@@ -635,9 +651,10 @@ void launch_push(Bufs& b, hipStream_t s)
     k_reinject<<<64, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.n_tree);
 }
 
-void launch_finish_step(Bufs& b, hipStream_t s)
+void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s)
 {
-    k_finish<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst);
+    k_finish<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst, compute_epe, b.epe_part, b.phi,
+                                            b.inv);
 }
 
 void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n, hipStream_t s)

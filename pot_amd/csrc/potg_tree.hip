@@ -415,11 +415,12 @@ __device__ inline void put32(NodeRec32* r32, uint32_t w, const InstParams& p, do
 }
 
 // leaf records: every tree particle is a leaf of charge CHARGE[species]
-// (the grain: its current charge, pot.c:1318-1328)
+// (the grain: its current charge, pot.c:1318-1328) and no dipole moment
 __global__ void k_leaf_records(PState st, const InstParams* __restrict__ P,
                                const InstState* __restrict__ S, uint32_t n_tree, uint32_t n_rec,
                                const uint32_t* __restrict__ wid_leaf, NodeRec64* __restrict__ r64,
-                               NodeRec32* __restrict__ r32)
+                               NodeRec32* __restrict__ r32, double* __restrict__ dip64,
+                               float4* __restrict__ dip32)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_tree) return;
@@ -437,11 +438,19 @@ __global__ void k_leaf_records(PState st, const InstParams* __restrict__ P,
     if (w >= n_rec) return;
     r64[w] = e;
     if (r32) put32(r32, w, P[inst], e.cx, e.cy, e.cz, 0.0, e.q, 0, LEAF, i);
+    if (dip64) {
+        dip64[3 * (size_t)w] = 0.0;
+        dip64[3 * (size_t)w + 1] = 0.0;
+        dip64[3 * (size_t)w + 2] = 0.0;
+    }
+    if (dip32) dip32[w] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// monopole moments of the internal nodes at one depth, children folded in
-// the reference order (compute_node_values / update_node_moments,
-// pot.c:1290-1402); the deeper levels were finished by earlier launches
+// moments of the internal nodes at one depth, children folded in the
+// reference order (compute_node_values / update_node_moments,
+// pot.c:1290-1402): charge, |charge|, centre of |charge| and, for DIPOLES
+// instances, the dipole moment about that centre; the deeper levels were
+// finished by earlier launches
 __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__ P,
                                 const InstState* __restrict__ S, const Globals* __restrict__ g,
                                 uint32_t cap, const int32_t* __restrict__ level,
@@ -450,9 +459,7 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
                                 const uint32_t* __restrict__ ccount,
                                 const uint32_t* __restrict__ cfirst,
                                 const uint32_t* __restrict__ wid_int,
-                                const double* __restrict__ size2, double* __restrict__ mcx,
-                                double* __restrict__ mcy, double* __restrict__ mcz,
-                                int32_t* __restrict__ mq, uint32_t* __restrict__ maq,
+                                const double* __restrict__ size2, NodeMoments m,
                                 NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
                                 uint32_t n_rec)
 {
@@ -461,7 +468,8 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
     if (k >= n_int_of(g, cap) || level[k] != L) return;
     const uint32_t inst = st.inst[owner[k]];
     const int32_t gq = S[inst].grain_q;
-    double cx = 0.0, cy = 0.0, cz = 0.0;
+    const bool dip = P[inst].dipoles && m.px;
+    double cx = 0.0, cy = 0.0, cz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     int q = 0;
     uint32_t aq = 0;
     for (int s = 0; s < 8; s++) {
@@ -469,7 +477,7 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
         if (c == NONE) continue;
         int qc;
         uint32_t aqc;
-        double ccx, ccy, ccz;
+        double ccx, ccy, ccz, cpx = 0.0, cpy = 0.0, cpz = 0.0;
         if (c & LEAF_BIT) {
             const uint32_t j = c & ~LEAF_BIT;
             qc = charge_of(st.spec[j], gq);
@@ -478,27 +486,47 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
             ccy = st.y[j];
             ccz = st.z[j];
         } else {
-            qc = mq[c];
-            aqc = maq[c];
-            ccx = mcx[c];
-            ccy = mcy[c];
-            ccz = mcz[c];
+            qc = m.q[c];
+            aqc = m.aq[c];
+            ccx = m.cx[c];
+            ccy = m.cy[c];
+            ccz = m.cz[c];
+            if (dip) {
+                cpx = m.px[c];
+                cpy = m.py[c];
+                cpz = m.pz[c];
+            }
         }
         if (aqc) {
-            // update_node_moments (pot.c:1297-1299), same float64 operations
+            // update_node_moments (pot.c:1290-1310), same float64 operations
+            const double ox = cx, oy = cy, oz = cz;
             const double den = (double)(aq + aqc);
             cx = ((double)aq * cx + (double)aqc * ccx) / den;
             cy = ((double)aq * cy + (double)aqc * ccy) / den;
             cz = ((double)aq * cz + (double)aqc * ccz) / den;
+            if (dip) {
+                // shift the dipole to the new centre, then add the child's
+                px -= (cx - ox) * (double)q;
+                py -= (cy - oy) * (double)q;
+                pz -= (cz - oz) * (double)q;
+                px += cpx - (cx - ccx) * (double)qc;
+                py += cpy - (cy - ccy) * (double)qc;
+                pz += cpz - (cz - ccz) * (double)qc;
+            }
             q += qc;
             aq += aqc;
         }
     }
-    mcx[k] = cx;
-    mcy[k] = cy;
-    mcz[k] = cz;
-    mq[k] = q;
-    maq[k] = aq;
+    m.cx[k] = cx;
+    m.cy[k] = cy;
+    m.cz[k] = cz;
+    m.q[k] = q;
+    m.aq[k] = aq;
+    if (dip) {
+        m.px[k] = px;
+        m.py[k] = py;
+        m.pz[k] = pz;
+    }
     NodeRec64 e;
     e.cx = cx;
     e.cy = cy;
@@ -512,6 +540,15 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
     if (w >= n_rec) return;
     r64[w] = e;
     if (r32) put32(r32, w, P[inst], cx, cy, cz, e.size2, q, e.first, e.cnt, aq);
+    if (m.dip64) {
+        m.dip64[3 * (size_t)w] = px;
+        m.dip64[3 * (size_t)w + 1] = py;
+        m.dip64[3 * (size_t)w + 2] = pz;
+    }
+    // the float32 walk works in units of R: p / R
+    if (m.dip32)
+        m.dip32[w] = make_float4((float)(px * P[inst].invR), (float)(py * P[inst].invR),
+                                 (float)(pz * P[inst].invR), 0.f);
 }
 
 }  // namespace
@@ -554,15 +591,16 @@ void launch_tree(Bufs& b, hipStream_t s)
     k_empty_roots<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.n_inst, b.rec64, b.precision ? b.rec32 : nullptr);
     if (n) {
         NodeRec32* r32 = b.precision ? b.rec32 : nullptr;
+        float4* d32 = b.precision ? b.dip32 : nullptr;
         k_leaf_records<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.n_rec, b.wid_leaf, b.rec64,
-                                              r32);
+                                              r32, b.dip64, d32);
+        NodeMoments m{b.mcx, b.mcy, b.mcz, b.mpx, b.mpy, b.mpz, b.mq, b.maq, b.dip64, d32};
         // deepest level first; launches past the tree's depth return at once
         for (int L = MAXD - 1; L >= 0; L--)
             k_level_moments<<<grid(b.cap_int), BT, 0, s>>>(L, b.st, b.P, b.S, b.g, b.cap_int,
                                                            b.level, b.owner, b.slots, b.ccount,
-                                                           b.cfirst, b.wid_int, b.size2, b.mcx,
-                                                           b.mcy, b.mcz, b.mq, b.maq, b.rec64, r32,
-                                                           b.n_rec);
+                                                           b.cfirst, b.wid_int, b.size2, m,
+                                                           b.rec64, r32, b.n_rec);
     }
 }
 

@@ -11,10 +11,10 @@
 //
 // Memory: the children of a node are contiguous records, so a stack entry
 // is a child range.  A range is fetched with one vector load (lane j holds
-// child j) and its children are then visited from registers, broadcast with
-// v_readlane -- one memory round trip per opened node instead of one per
-// child.  Ranges interrupted by a descent are pushed back (remaining
-// siblings, same mask) on an LDS stack whose depth is the tree depth.
+// child j) and its children are then visited from registers or LDS -- one
+// memory round trip per opened node instead of one per child.  Ranges
+// interrupted by a descent are pushed back (remaining siblings, same mask)
+// on an LDS stack.
 //
 // FP64: every operation repeats the reference's float64 expressions, with
 // float64 instead of x87 accumulation.  MIXED: coordinates relative to the
@@ -32,8 +32,13 @@ constexpr int BT = 256;
 constexpr int NW = BT / WAVE;
 constexpr int STK = MAXD + 4;
 
-enum { KIND_FIELD = 0, KIND_PHI = 1 };
-enum { MODE_STEP = 0, MODE_POINTS = 1 };
+// KIND_EPE: the potential a particle feels, its own leaf skipped
+// (phi_particle_feels, pot.c:1934-2018); KIND_PHI: the potential at a
+// point, every leaf counted (phi_at, pot.c:1818-1901)
+enum { KIND_FIELD = 0, KIND_PHI = 1, KIND_EPE = 2 };
+// MODE_EPE: every particle slot, at the position the potential-energy sum
+// reads (half-drifted when stepped this step, pot.c:2478-2480, 2784-2792)
+enum { MODE_STEP = 0, MODE_POINTS = 1, MODE_EPE = 2 };
 
 struct WalkIO {
     const uint32_t* targets;
@@ -46,6 +51,9 @@ struct WalkIO {
     uint32_t* counts;
     uint32_t* tile_ctr;
     uint32_t n_rec;    // record capacity: child ranges must stay inside it
+    const InstState* S;
+    const double* dip64;   // record dipoles (DIPOLES instances), else null
+    const float4* dip32;
 };
 
 __device__ inline void kahan_add(float& s, float& c, float v)
@@ -90,6 +98,15 @@ __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, 
         tx = st.x[i] + (0.5 * dts) * st.vx[i];
         ty = st.y[i] + (0.5 * dts) * st.vy[i];
         tz = st.z[i] + (0.5 * dts) * st.vz[i];
+    } else if (MODE == MODE_EPE) {
+        self = k;
+        inst = st.inst[k];
+        const int sp = st.spec[k];
+        const bool stepped = sp != 0 && st.ref[k] - P[inst].offset >= io.S[inst].start;
+        const double hdt = stepped ? 0.5 * (P[inst].dt * (double)P[inst].tss[sp]) : 0.0;
+        tx = stepped ? st.x[k] + hdt * st.vx[k] : st.x[k];
+        ty = stepped ? st.y[k] + hdt * st.vy[k] : st.y[k];
+        tz = stepped ? st.z[k] + hdt * st.vz[k] : st.z[k];
     } else {
         inst = io.pt_inst[k];
         self = io.pt_self[k];
@@ -142,6 +159,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
             const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
+            const bool dip = P[gi].dipoles && io.dip64;
             uint32_t tf = gi, tc = 1;
             uint64_t tm = grp;
             int sp = 0;
@@ -154,15 +172,21 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
                     tc = rfl(s_cnt[wv][sp]);
                     tm = rfl64(s_mask[wv][sp]);
                 }
-                if (++loads > io.n_rec) {
-                    raise_err(g, KERR_DEPTH, tf);
+                if (++loads > io.n_rec || tc > 8) {
+                    raise_err(g, tc > 8 ? KERR_CAPACITY : KERR_DEPTH, tf);
                     break;
                 }
                 tot_pops++;
                 // one load for the whole range: lane j holds child j
                 NodeRec64 mine;
+                double mpx = 0.0, mpy = 0.0, mpz = 0.0;
                 if (lane < tc) mine = rec[tf + lane];
                 else mine = NodeRec64{0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0};
+                if (dip && lane < tc) {
+                    mpx = io.dip64[3 * (size_t)(tf + lane)];
+                    mpy = io.dip64[3 * (size_t)(tf + lane) + 1];
+                    mpz = io.dip64[3 * (size_t)(tf + lane) + 2];
+                }
                 const bool act = (tm >> lane) & 1ull;
                 uint32_t next_first = 0, next_cnt = 0;
                 uint64_t next_mask = 0;
@@ -191,13 +215,35 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
                                         raise_err(g, KERR_COINCIDENT, self);
                                     }
                                 }
-                            } else {
+                            } else if (KIND == KIND_PHI || bcast_u(mine.part, j) != self) {
                                 ph += (QE * q) / (K4PE0 * sqrt(d2 + soft2));
                                 nleaf++;
                             }
                         } else if (bcast_d(mine.size2, j) < theta2 * d2) {
-                            if (q) {
-                                if (KIND == KIND_FIELD) {
+                            // DIPOLES accepts every cell holding charge, net
+                            // or not (pot.c:1732-1736); internal records
+                            // carry |q| in `part`
+                            if (dip ? bcast_u(mine.part, j) != 0 : q != 0) {
+                                if (dip) {
+                                    const double px = bcast_d(mpx, j), py = bcast_d(mpy, j),
+                                                 pz = bcast_d(mpz, j);
+                                    const double dist = sqrt(d2);
+                                    if (KIND == KIND_FIELD) {
+                                        // pot.c:1738-1745
+                                        const double es = QE / ((K4PE0 * d2) * dist);
+                                        const double rpd2 = ((dx * px) + (dy * py) + (dz * pz)) / d2;
+                                        ex += es * ((q * dx) - px + ((3 * dx) * rpd2));
+                                        ey += es * ((q * dy) - py + ((3 * dy) * rpd2));
+                                        ez += es * ((q * dz) - pz + ((3 * dz) * rpd2));
+                                    } else {
+                                        const double rdp = (dx * px) + (dy * py) + (dz * pz);
+                                        const double t = (q / dist) + (rdp / (dist * d2));
+                                        // phi_at (pot.c:1862) and phi_particle_feels
+                                        // (pot.c:1969) group the constants differently
+                                        if (KIND == KIND_PHI) ph += (QE / K4PE0) * t;
+                                        else ph += (QE * t) / K4PE0;
+                                    }
+                                } else if (KIND == KIND_FIELD) {
                                     const double dist = sqrt(d2);
                                     double es = QE / ((K4PE0 * d2) * dist);
                                     es *= q;
@@ -279,20 +325,87 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
     }
 }
 
-// ---- float32 compensated walk (field) ---------------------------------
-// The current child range is staged in LDS (one record per lane, written by
-// lanes 0..cnt-1) and each child is read back by every lane with two
-// broadcast ds_read_b128; the interactions of one range are summed in plain
-// float32 and the range total enters the per-particle Kahan sum.
+// ---- float32 compensated walk (field): child steps or target x child pairs
+//
+// Same traversal and per-particle MAC as k_walk64, in float32 with
+// coordinates relative to the domain centre in units of R.  A fetched child
+// range (<= 8 children, staged in LDS) is evaluated against the targets in
+// its mask in one of two ways:
+//   * child steps (dense masks): the wave visits the children one at a
+//     time, youngest first, each lane testing its own target;
+//   * pairs (sparse masks, at most `pair_max` targets): the m x C
+//     (target, child) pairs, C = cnt rounded up to a power of two, are laid
+//     out densely across the lanes, so a range costs ceil(m C / 64) wave
+//     operations instead of cnt.  Contributions are added per target in
+//     LDS, and every child opened by some targets is pushed with the mask
+//     of exactly those targets, youngest child on top.
+// Either way each target gets exactly the reference's interaction set;
+// only the summation order differs from the depth-first order.  The
+// interactions of one range visit are summed in plain float32 and that sum
+// enters the per-particle Kahan sum.
+constexpr int STKH = 192;
+
+// one (target, node) interaction; returns true when the target opens the node.
+// With dip (DIPOLES), cells holding any charge are accepted and pd is the
+// cell's dipole in units of R (pot.c:1732-1745).
+__device__ inline bool interact32(float tx, float ty, float tz, uint32_t self, float theta2,
+                                  float soft2, const NodeRec32& r, bool dip, float4 pd, float& ax,
+                                  float& ay, float& az, uint32_t& ncell, uint32_t& nleaf)
+{
+    const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    if (r.cnt == LEAF) {
+        if (r.part != self && d2 != 0.f) {
+            const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
+            const float sc = r.q * (inv * inv * inv);
+            ax = __builtin_fmaf(sc, dx, ax);
+            ay = __builtin_fmaf(sc, dy, ay);
+            az = __builtin_fmaf(sc, dz, az);
+            nleaf++;
+        }
+        return false;
+    }
+    if (r.size2 < theta2 * d2) {
+        if (dip ? r.part != 0u : r.q != 0.f) {
+            const float inv = __builtin_amdgcn_rsqf(d2);
+            const float inv2 = inv * inv;
+            const float inv3 = inv2 * inv;
+            if (dip) {
+                const float rpd2 =
+                    __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
+                const float t3 = 3.f * rpd2;
+                ax = __builtin_fmaf(inv3, __builtin_fmaf(r.q + t3, dx, -pd.x), ax);
+                ay = __builtin_fmaf(inv3, __builtin_fmaf(r.q + t3, dy, -pd.y), ay);
+                az = __builtin_fmaf(inv3, __builtin_fmaf(r.q + t3, dz, -pd.z), az);
+            } else {
+                const float sc = r.q * inv3;
+                ax = __builtin_fmaf(sc, dx, ax);
+                ay = __builtin_fmaf(sc, dy, ay);
+                az = __builtin_fmaf(sc, dz, az);
+            }
+            ncell++;
+        }
+        return false;
+    }
+    return r.cnt != 0;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec32* __restrict__ rec,
-                                                WalkIO io)
+                                                WalkIO io, uint32_t pair_max)
 {
-    __shared__ uint32_t s_first[NW][STK];
-    __shared__ uint32_t s_cnt[NW][STK];
-    __shared__ uint64_t s_mask[NW][STK];
-    __shared__ float4 s_rec[NW][8][2];
+    __shared__ uint32_t s_first[NW][STKH];
+    __shared__ uint32_t s_cnt[NW][STKH];
+    __shared__ uint64_t s_mask[NW][STKH];
+    __shared__ NodeRec32 s_rec[NW][8];
+    __shared__ float4 s_dip[NW][8];
+    __shared__ float s_tx[NW][WAVE], s_ty[NW][WAVE], s_tz[NW][WAVE];
+    __shared__ uint32_t s_self[NW][WAVE];
+    __shared__ float s_ax[NW][WAVE], s_ay[NW][WAVE], s_az[NW][WAVE];
+    __shared__ uint32_t s_packed[NW][WAVE];  // cell | leaf << 10 | open << 20
+    __shared__ uint8_t s_lane[NW][WAVE];
+    __shared__ unsigned long long s_omask[NW][8];
     const int wv = threadIdx.x / WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
@@ -313,6 +426,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         const float tz = (float)((dtz - p.R) * p.invR);
         const float theta2 = (float)p.theta2;
         const float soft2 = (float)(p.soft2 * p.invR * p.invR);
+        s_tx[wv][lane] = tx;
+        s_ty[wv][lane] = ty;
+        s_tz[wv][lane] = tz;
+        s_self[wv][lane] = self;
         float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         uint32_t ncell = 0, nleaf = 0, nopen = 0;
 
@@ -322,6 +439,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
             const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
+            const bool dip = P[gi].dipoles && io.dip32;
             uint32_t tf = gi, tc = 1;
             uint64_t tm = grp;
             int sp = 0;
@@ -334,59 +452,97 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     tc = rfl(s_cnt[wv][sp]);
                     tm = rfl64(s_mask[wv][sp]);
                 }
-                if (++loads > io.n_rec) {
-                    raise_err(g, KERR_DEPTH, tf);
+                if (++loads > io.n_rec || tc > 8) {
+                    raise_err(g, tc > 8 ? KERR_CAPACITY : KERR_DEPTH, tf);
                     break;
                 }
                 tot_pops++;
                 if (lane < tc) {
-                    const float4* src = reinterpret_cast<const float4*>(rec + tf + lane);
-                    s_rec[wv][lane][0] = src[0];
-                    s_rec[wv][lane][1] = src[1];
+                    s_rec[wv][lane] = rec[tf + lane];
+                    if (dip) s_dip[wv][lane] = io.dip32[tf + lane];
                 }
                 // the wave reads what its own lanes just wrote: LDS keeps a
                 // wave's accesses in order; keep the compiler from moving them
                 __builtin_amdgcn_wave_barrier();
+                const uint32_t m = __popcll(tm);
+                // a pair step pushes up to tc ranges; child steps push at most
+                // one per level below, so MAXD + 1 entries stay in reserve
+                if (m <= pair_max && sp + (int)tc + MAXD + 1 <= STKH) {
+                    const uint32_t lsh = tc > 4 ? 3 : (tc > 2 ? 2 : (tc > 1 ? 1 : 0));
+                    if ((tm >> lane) & 1ull)
+                        s_lane[wv][__popcll(tm & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+                    s_ax[wv][lane] = 0.f;
+                    s_ay[wv][lane] = 0.f;
+                    s_az[wv][lane] = 0.f;
+                    s_packed[wv][lane] = 0u;
+                    if (lane < 8) s_omask[wv][lane] = 0ull;
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t total = m << lsh;
+                    for (uint32_t base = 0; base < total; base += WAVE) {
+                        const uint32_t pi = base + lane;
+                        const uint32_t c = pi & ((1u << lsh) - 1u);
+                        if (pi < total && c < tc) {
+                            const uint32_t t = s_lane[wv][pi >> lsh];
+                            float ax = 0.f, ay = 0.f, az = 0.f;
+                            uint32_t cc = 0, cl = 0;
+                            const float4 pd = dip ? s_dip[wv][c] : make_float4(0.f, 0.f, 0.f, 0.f);
+                            const bool op = interact32(s_tx[wv][t], s_ty[wv][t], s_tz[wv][t],
+                                                       s_self[wv][t], theta2, soft2, s_rec[wv][c],
+                                                       dip, pd, ax, ay, az, cc, cl);
+                            if (cc | cl) {
+                                atomicAdd(&s_ax[wv][t], ax);
+                                atomicAdd(&s_ay[wv][t], ay);
+                                atomicAdd(&s_az[wv][t], az);
+                            }
+                            if (op) atomicOr(&s_omask[wv][c], 1ull << t);
+                            const uint32_t pk = cc | (cl << 10) | ((op ? 1u : 0u) << 20);
+                            if (pk) atomicAdd(&s_packed[wv][t], pk);
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t pk = s_packed[wv][lane];
+                    ncell += pk & 1023u;
+                    nleaf += (pk >> 10) & 1023u;
+                    nopen += pk >> 20;
+                    kahan_add(fx, kx, s_ax[wv][lane]);
+                    kahan_add(fy, ky, s_ay[wv][lane]);
+                    kahan_add(fz, kz, s_az[wv][lane]);
+                    // push the opened children, the youngest last (visited first)
+                    bool fail = false;
+                    for (uint32_t c = 0; c < tc; c++) {
+                        const uint64_t om = rfl64(s_omask[wv][c]);
+                        if (!om) continue;
+                        const uint32_t cf = rfl(s_rec[wv][c].first), cn = rfl(s_rec[wv][c].cnt);
+                        if (cf <= tf + c || cf + cn > io.n_rec) {
+                            raise_err(g, KERR_CAPACITY, tf + c);
+                            fail = true;
+                            break;
+                        }
+                        s_first[wv][sp] = cf;
+                        s_cnt[wv][sp] = cn;
+                        s_mask[wv][sp] = om;
+                        sp++;
+                    }
+                    if (fail) break;
+                    tc = 0;
+                    continue;
+                }
                 const bool act = (tm >> lane) & 1ull;
-                float px = 0.f, py = 0.f, pz = 0.f;  // this range's float32 partial
+                float px = 0.f, py = 0.f, pz = 0.f;  // this range visit's float32 partial
                 uint32_t next_first = 0, next_cnt = 0;
                 uint64_t next_mask = 0;
                 int j = (int)tc - 1;
                 for (; j >= 0; j--) {
-                    const float4 a = s_rec[wv][j][0];  // cx, cy, cz, size2
-                    const float4 b = s_rec[wv][j][1];  // q, first, cnt, part
-                    const uint32_t cnt = rfl(__float_as_uint(b.z));
-                    bool open = false;
-                    if (act) {
-                        const float dx = tx - a.x, dy = ty - a.y, dz = tz - a.z;
-                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        if (cnt == LEAF) {
-                            if (__float_as_uint(b.w) != self && d2 != 0.f) {
-                                const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
-                                const float s = b.x * (inv * inv * inv);
-                                px = __builtin_fmaf(s, dx, px);
-                                py = __builtin_fmaf(s, dy, py);
-                                pz = __builtin_fmaf(s, dz, pz);
-                                nleaf++;
-                            }
-                        } else if (a.w < theta2 * d2) {
-                            if (b.x != 0.f) {
-                                const float inv = __builtin_amdgcn_rsqf(d2);
-                                const float s = b.x * (inv * inv * inv);
-                                px = __builtin_fmaf(s, dx, px);
-                                py = __builtin_fmaf(s, dy, py);
-                                pz = __builtin_fmaf(s, dz, pz);
-                                ncell++;
-                            }
-                        } else {
-                            open = cnt != 0;
-                        }
-                    }
-                    const uint64_t om = rfl64(ballot(open));
+                    const NodeRec32 r = s_rec[wv][j];
+                    const float4 pd = dip ? s_dip[wv][j] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    bool op = false;
+                    if (act) op = interact32(tx, ty, tz, self, theta2, soft2, r, dip, pd, px, py,
+                                             pz, ncell, nleaf);
+                    const uint64_t om = rfl64(ballot(op));
                     if (om) {
-                        if (open) nopen++;
-                        next_first = rfl(__float_as_uint(b.y));
-                        next_cnt = cnt;
+                        if (op) nopen++;
+                        next_first = rfl(r.first);
+                        next_cnt = rfl(r.cnt);
                         next_mask = om;
                         break;
                     }
@@ -403,7 +559,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     break;
                 }
                 if (j > 0) {
-                    if (sp >= STK) {
+                    if (sp >= STKH) {
                         raise_err(g, KERR_DEPTH, tf);
                         break;
                     }
@@ -446,9 +602,11 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
 }
 
 // initial accumulator of every target: the spheroid grain's analytic field
-// (SPH:1815-1818 seeds E with it) or, for the potential, its value at the
-// absolute coordinates as phi_at uses them (SPH:1924-1953); zero for the
-// spherical grain, which is a leaf of the tree
+// (SPH:1815-1818 seeds E with it); for phi_at, the grain potential at the
+// absolute coordinates as phi_at uses them (SPH:1924-1953); for the
+// potential a particle feels, the grain potential at its grain-relative
+// position (SPH:2040-2084, 2100-2103).  Zero for the spherical grain, which
+// is a leaf of the tree.
 template <int MODE, int KIND>
 __global__ void k_grain_init(const InstParams* __restrict__ P, const InstState* __restrict__ S,
                              PState st, WalkIO io)
@@ -465,8 +623,15 @@ __global__ void k_grain_init(const InstParams* __restrict__ P, const InstState* 
         io.out[3 * k] = e[0];
         io.out[3 * k + 1] = e[1];
         io.out[3 * k + 2] = e[2];
-    } else {
+    } else if (KIND == KIND_PHI) {
         io.out[k] = p.geometry != 0 ? grain_phi_abs(p, S[inst].grain_q, tx, ty, tz) : 0.0;
+    } else {
+        // grain-relative, and nothing for the grain itself (SPH:2040-2050)
+        const bool grain = st.ref[k] == p.offset;
+        io.out[k] = (p.geometry != 0 && !grain)
+                        ? grain_phi_abs(p, S[inst].grain_q, tx - p.grain[0], ty - p.grain[1],
+                                        tz - p.grain[2])
+                        : 0.0;
     }
 }
 
@@ -496,6 +661,8 @@ static WalkIO step_io(Bufs& b)
     io.out = b.E;
     io.tile_ctr = &b.g->tile_ctr;
     io.n_rec = b.n_rec;
+    io.dip64 = b.dip64;
+    io.dip32 = b.dip32;
     return io;
 }
 
@@ -510,7 +677,7 @@ void launch_walk(Bufs& b, hipStream_t s)
 {
     WalkIO io = step_io(b);
     if (b.precision)
-        k_walk32<MODE_STEP><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io);
+        k_walk32<MODE_STEP><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
     else
         k_walk64<MODE_STEP, KIND_FIELD><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
 }
@@ -526,6 +693,8 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     io.counts = kind == 0 ? b.q_counts : nullptr;
     io.tile_ctr = &b.g->tile_ctr2;
     io.n_rec = b.n_rec;
+    io.dip64 = b.dip64;
+    io.dip32 = b.dip32;
     k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr2);
     const uint32_t nb = (n + BT - 1) / BT ? (n + BT - 1) / BT : 1;
     const int blocks = (int)nb < walk_grid() ? (int)nb : walk_grid();
@@ -535,10 +704,62 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     } else {
         k_grain_init<MODE_POINTS, KIND_FIELD><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
         if (b.precision)
-            k_walk32<MODE_POINTS><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io);
+            k_walk32<MODE_POINTS><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
         else
             k_walk64<MODE_POINTS, KIND_FIELD><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
     }
+}
+
+// Potential energies of a compute_epe step (pot.c:2771-2792): the potential
+// each particle feels, from the step's tree, at its half-drifted position,
+// before the push moves it; then deterministic per-instance partial sums of
+// QE * CHARGE * phi over electrons and ions (EPE_BLOCKS per instance, added
+// in a fixed order by the step's last kernel).
+__global__ __launch_bounds__(BT) void k_epe_partial(const InstParams* __restrict__ P, PState st,
+                                                    const double* __restrict__ phi,
+                                                    double* __restrict__ part)
+{
+    const uint32_t inst = blockIdx.y;
+    const InstParams& p = P[inst];
+    double se = 0.0, si = 0.0;
+    for (uint32_t j = blockIdx.x * BT + threadIdx.x; j < p.tree_n; j += EPE_BLOCKS * BT) {
+        const uint32_t k = p.tree_begin + j;
+        const int sp = st.spec[k];
+        if (sp == 2) se += -QE * phi[k];
+        else if (sp == 1) si += QE * phi[k];
+    }
+    __shared__ double red[2][BT];
+    red[0][threadIdx.x] = se;
+    red[1][threadIdx.x] = si;
+    __syncthreads();
+    for (int w = BT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (inst * EPE_BLOCKS + blockIdx.x)] = red[0][0];
+        part[2 * (inst * EPE_BLOCKS + blockIdx.x) + 1] = red[1][0];
+    }
+}
+
+void launch_epe(Bufs& b, hipStream_t s)
+{
+    WalkIO io{};
+    io.n_pts = b.n_total;
+    io.out = b.phi;
+    io.tile_ctr = &b.g->tile_ctr2;
+    io.n_rec = b.n_rec;
+    io.S = b.S;
+    io.dip64 = b.dip64;
+    k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr2);
+    const uint32_t nb = (b.n_total + BT - 1) / BT;
+    const int blocks = (int)nb < walk_grid() ? (int)nb : walk_grid();
+    k_grain_init<MODE_EPE, KIND_EPE><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
+    k_walk64<MODE_EPE, KIND_EPE><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    k_epe_partial<<<dim3(EPE_BLOCKS, b.n_inst), BT, 0, s>>>(b.P, b.st, b.phi, b.epe_part);
 }
 
 // chi_b every 20 steps (pot.c:2409-2418): the potential at one boundary
@@ -546,7 +767,8 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
 // of nodes at the domain edge)
 __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, InstState* S,
                                               Globals* g, const NodeRec64* __restrict__ r64,
-                                              uint32_t n_inst, uint32_t n_rec)
+                                              const double* __restrict__ dip64, uint32_t n_inst,
+                                              uint32_t n_rec)
 {
     if ((g->t % 20) != 0) return;
     const uint32_t inst = blockIdx.x;
@@ -571,7 +793,16 @@ __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, 
         if (r.cnt == LEAF) {
             ph += (QE * r.q) / (K4PE0 * sqrt(d2 + soft2));
         } else if (r.size2 < theta2 * d2) {
-            if (r.q) ph += (QE * r.q) / (K4PE0 * sqrt(d2));
+            if (p.dipoles && dip64) {
+                if (r.part) {  // |q| of the cell (pot.c:1856-1862)
+                    const double* d = dip64 + 3 * (size_t)w;
+                    const double dist = sqrt(d2);
+                    const double rdp = (dx * d[0]) + (dy * d[1]) + (dz * d[2]);
+                    ph += (QE / K4PE0) * ((r.q / dist) + (rdp / (dist * d2)));
+                }
+            } else if (r.q) {
+                ph += (QE * r.q) / (K4PE0 * sqrt(d2));
+            }
         } else {
             if (sp + (int)r.cnt > 8 * STK || (r.cnt && r.first <= w) || r.first + r.cnt > n_rec) {
                 raise_err(g, KERR_DEPTH, w);
@@ -585,7 +816,7 @@ __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, 
 
 void launch_chi_b(Bufs& b, hipStream_t s)
 {
-    k_chi_b<<<b.n_inst, 64, 0, s>>>(b.P, b.S, b.g, b.rec64, b.n_inst, b.n_rec);
+    k_chi_b<<<b.n_inst, 64, 0, s>>>(b.P, b.S, b.g, b.rec64, b.dip64, b.n_inst, b.n_rec);
 }
 
 }  // namespace potg

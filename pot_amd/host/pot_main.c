@@ -18,6 +18,7 @@
  *   --bx/--by/--bz X EXTERNAL_B_{X,Y,Z} in tesla (pot.c:55-57)
  *   --mime X         MIME, ion/electron mass ratio (pot.c:60)
  *   --no-grain       DISABLE_GRAIN (pot.c:30)
+ *   --dipoles        DIPOLES: dipole moments in the tree (pot.c:99-105)
  *   --oblate         OBLATE instead of PROLATE (pot-spheroid only)
  *   --fp64           float64 walk (default: float32 walk with compensated
  *                    sums, float64 push)
@@ -103,7 +104,7 @@ static void usage(const char* prog)
           "  -x [NUMBER]   drift velocity in x-direction\n",
           stderr);
     if (SPHEROID_PROGRAM) fputs("  -A [NUMBER]   aspect ratio of the spheroid\n", stderr);
-    fputs("  --theta X  --bx X --by X --bz X  --mime X  --no-grain  --fp64\n"
+    fputs("  --theta X  --bx X --by X --bz X  --mime X  --no-grain  --dipoles  --fp64\n"
           "  --seed-file PATH  --output-dir DIR", stderr);
     fputs(SPHEROID_PROGRAM ? "  --oblate\n" : "\n", stderr);
 }
@@ -126,6 +127,10 @@ static int parse_args(int argc, char** argv, pot_run* ru)
             const char* v = NULL;
             if (!strcmp(o, "--no-grain")) {
                 ru->grain = 0;
+                continue;
+            }
+            if (!strcmp(o, "--dipoles")) {
+                ru->dipoles = 1;
                 continue;
             }
             if (!strcmp(o, "--fp64")) {
@@ -384,6 +389,7 @@ static potg_config device_config(const pot_run* ru, uint64_t key)
     c.grain_enabled = ru->grain;
     c.sceptic = 1;
     c.seed = key;
+    c.dipoles = ru->dipoles;
     return c;
 }
 

@@ -26,7 +26,7 @@ ERRORS = {-1: "POTG_ERR_ARG", -2: "POTG_ERR_HIP", -3: "POTG_ERR_OUT_OF_BOX",
 EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "potg_download",
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
-           "potg_stage_times", "potg_sync"]
+           "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles"]
 
 
 class Config(C.Structure):
@@ -35,7 +35,8 @@ class Config(C.Structure):
                 ("x_drift", C.c_double), ("a", C.c_double), ("settle", C.c_double),
                 ("Te", C.c_double), ("Ti", C.c_double), ("A", C.c_double), ("theta", C.c_double),
                 ("B", C.c_double * 3), ("max_lin_dev", C.c_double), ("mime", C.c_double),
-                ("grain_enabled", C.c_int32), ("sceptic", C.c_int32), ("seed", C.c_uint64)]
+                ("grain_enabled", C.c_int32), ("sceptic", C.c_int32), ("seed", C.c_uint64),
+                ("dipoles", C.c_int32), ("pad", C.c_int32)]
 
 
 class Grain(C.Structure):
@@ -91,6 +92,7 @@ def lib():
     L.potg_stats_reset.argtypes = [P]
     L.potg_stage_times.argtypes = [P, C.c_int32, dp]
     L.potg_sync.argtypes = [P]
+    L.potg_tree_canon_dipoles.argtypes = [P, C.c_uint32, P, C.c_int64, C.POINTER(C.c_int64)]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
@@ -113,7 +115,7 @@ def _up(a):
 
 def config(N, r, mpp, dt, soft, a, Te, Ti, x_drift=0.0, settle=0.0, geometry=POTG_SPHERE,
            A=1.0, theta=0.5, B=(0.0, 0.0, 0.0), seed=1, grain_enabled=1, sceptic=1,
-           mime=None, max_lin_dev=None):
+           mime=None, max_lin_dev=None, dipoles=0):
     c = Config()
     c.N, c.r, c.geometry = int(N), int(r), int(geometry)
     c.dt, c.mpp, c.soft, c.x_drift, c.a, c.settle = dt, mpp, soft, x_drift, a, settle
@@ -123,6 +125,7 @@ def config(N, r, mpp, dt, soft, a, Te, Ti, x_drift=0.0, settle=0.0, geometry=POT
     c.max_lin_dev = max_lin_dev if max_lin_dev is not None else (0.1 if sph else 0.01)
     c.mime = mime if mime is not None else (1836.153 if sph else 1836.15)
     c.grain_enabled, c.sceptic, c.seed = grain_enabled, sceptic, seed
+    c.dipoles = dipoles
     return c
 
 
@@ -199,6 +202,13 @@ class Context:
         n = n.value
         return {"depth": depth[:n], "is_leaf": leaf[:n], "part": part[:n], "q": q[:n],
                 "aq": aq[:n], "c": c[:n], "size2": s2[:n], "nchild": nch[:n]}
+
+    def tree_dipoles(self, inst=0):
+        cap = 4 * int(self.configs[inst].N) + 16
+        p = np.zeros((cap, 3))
+        n = C.c_int64()
+        _check(self.L.potg_tree_canon_dipoles(self.h, inst, p.ctypes.data, cap, C.byref(n)))
+        return p[:n.value]
 
     def field_at(self, inst, xyz, exclude=None, counts=False):
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)

@@ -42,7 +42,8 @@ def gpu_config(case, seed=0):
                     Te=L.or_strtold_to_double(a["-E"].encode()),
                     Ti=L.or_strtold_to_double(a["-I"].encode()),
                     x_drift=float(a.get("-x", "0")), settle=0.0, geometry=case["geometry"],
-                    A=float(a.get("-A", "1")), theta=case["theta"], B=tuple(case["B"]), seed=seed)
+                    A=float(a.get("-A", "1")), theta=case["theta"], B=tuple(case["B"]), seed=seed,
+                    dipoles=case.get("dipoles", 0))
 
 
 def gpu_ctx(case, parts, precision=G.POTG_FP64, seed=0, q=0, t=0, chi_b=0.0):

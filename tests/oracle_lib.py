@@ -83,6 +83,7 @@ def lib():
         "or_build_tree": (C.c_int, [P]),
         "or_tree_canon": (C.c_int64, [P, P, P, P, P, P, P, P, P, C.c_int64]),
         "or_tree_used": (C.c_uint32, [P]),
+        "or_tree_canon_dipoles": (C.c_int64, [P, dp, C.c_int64]),
         "or_field_at": (None, [P, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_int32,
                                dp, C.POINTER(WalkStats)]),
         "or_phi_at": (C.c_double, [P, C.c_double, C.c_double, C.c_double]),
@@ -216,6 +217,14 @@ class Sim:
             raise RuntimeError("tree_canon failed")
         return {"depth": depth[:n], "is_leaf": leaf[:n], "part": part[:n], "q": q[:n],
                 "aq": aq[:n], "c": c[:n], "x1x2": x12[:n], "nchild": nch[:n]}
+
+    def tree_dipoles(self):
+        cap = 4 * self.N + 16
+        p = np.zeros((cap, 3))
+        n = self.L.or_tree_canon_dipoles(self.h, dptr(p), cap)
+        if n < 0:
+            raise RuntimeError("tree_canon_dipoles failed")
+        return p[:n]
 
     def field_at(self, part, x, y, z, accum_double=0, stats=None):
         E = np.zeros(3)

@@ -20,7 +20,8 @@ from pot_amd import potg as G
 
 SMALL = [c for c in CASES]
 BIG = [big_case(20001), big_case(20001, geometry=1, A=2.0, seed=12),
-       big_case(20001, geometry=2, A=0.5, seed=13)]
+       big_case(20001, geometry=2, A=0.5, seed=13),
+       dict(big_case(20001, seed=14), name="big20001_dipoles", dipoles=1)]
 ALL = SMALL + BIG
 IDS = [c["name"] for c in ALL]
 
@@ -47,6 +48,9 @@ def test_tree_bit_exact(case):
     internal = ref["is_leaf"] == 0
     size2 = (ref["x1x2"][:, 1] - ref["x1x2"][:, 0]) ** 2
     assert np.array_equal(got["size2"][internal], size2[internal])
+    if case.get("dipoles"):
+        # the dipole fold of update_node_moments (pot.c:1300-1307), bit for bit
+        assert np.array_equal(ctx.tree_dipoles(0), sim.tree_dipoles())
 
 
 @pytest.mark.parametrize("case", ALL, ids=IDS)
