@@ -118,6 +118,15 @@ typedef struct potg_stats_t {
     uint64_t n_range_loads; /* child ranges fetched by walk wavefronts */
 } potg_stats_t;
 
+/* Launch parameters of the field walk (potg_tune picks them by timing). */
+typedef struct potg_tuning {
+    uint32_t pair_max;    /* child ranges whose mask holds at most this many
+                             targets are evaluated as target x child pairs
+                             (0..64; float32 walk only) */
+    uint32_t walk_blocks; /* blocks (4 wavefronts each) of the persistent walk */
+    double walk_ms;       /* potg_tune: time of the chosen walk on the next step */
+} potg_tuning;
+
 typedef struct potg_ctx potg_ctx;
 
 /* Create a context for n_inst instances (configs[i]); precision is
@@ -176,6 +185,14 @@ int potg_phi_at(potg_ctx* ctx, uint32_t inst, const double* xyz, size_t n, doubl
    instance inst over all its other particles (E_ref of the force checks). */
 int potg_direct_sum(potg_ctx* ctx, uint32_t inst, const uint32_t* idx, size_t n,
                     double* E);
+
+/* Time the field walk of the coming step under candidate launch parameters
+   (pair threshold, grid size), keep the fastest and report it in *out.  The
+   step itself is not taken: the state, step counter and work counters are
+   left as they were.  No counterpart in the reference. */
+int potg_tune(potg_ctx* ctx, potg_tuning* out);
+/* Set or read the walk's launch parameters (set NULL to only read). */
+int potg_set_tuning(potg_ctx* ctx, const potg_tuning* set, potg_tuning* get);
 
 int potg_stats(potg_ctx* ctx, potg_stats_t* out);
 int potg_stats_reset(potg_ctx* ctx);

@@ -56,6 +56,7 @@ struct potg_ctx {
     void* d_aos;
     size_t aos_cap;
     std::vector<char> uploaded;
+    double tuned_ms = 0.0;
 };
 
 namespace {
@@ -305,13 +306,10 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
     b.key_bits = 64;
-    // walk ranges with at most this many targets in the mask are evaluated
-    // as target x child pairs; POTG_WALK_PAIR_MAX overrides it (0..64)
+    // defaults until potg_tune or potg_set_tuning: ranges with at most 16
+    // targets go pair-parallel; 8 blocks of 4 waves per CU
     b.pair_max = 16;
-    if (const char* e = getenv("POTG_WALK_PAIR_MAX")) {
-        const long v = strtol(e, nullptr, 10);
-        if (v >= 0 && v <= 64) b.pair_max = (uint32_t)v;
-    }
+    b.walk_blocks = 8 * device_cus();
 
     c->hp.resize(n_inst);
     uint32_t off = 0, tb = 0;
@@ -756,6 +754,94 @@ int potg_stats_reset(potg_ctx* c)
     g.max_depth = 0;
     HIPCHK(hipMemcpy(c->b.g, &g, sizeof(g), hipMemcpyHostToDevice));
     return POTG_OK;
+}
+
+int potg_set_tuning(potg_ctx* c, const potg_tuning* set, potg_tuning* get)
+{
+    if (!c) return set_err("potg_set_tuning: null context"), POTG_ERR_ARG;
+    if (set) {
+        if (set->pair_max > 64 || set->walk_blocks == 0 || set->walk_blocks > 65536)
+            return set_err("potg_set_tuning: pair_max must be 0..64, walk_blocks 1..65536"),
+                   POTG_ERR_ARG;
+        c->b.pair_max = set->pair_max;
+        c->b.walk_blocks = set->walk_blocks;
+        destroy_graphs(c);  // captured steps hold the old launch parameters
+    }
+    if (get) {
+        get->pair_max = c->b.pair_max;
+        get->walk_blocks = c->b.walk_blocks;
+        get->walk_ms = c->tuned_ms;
+    }
+    return POTG_OK;
+}
+
+int potg_tune(potg_ctx* c, potg_tuning* out)
+{
+    if (!c) return set_err("potg_tune: null context"), POTG_ERR_ARG;
+    if (all_uploaded(c)) return POTG_ERR_ARG;
+    Bufs& b = c->b;
+    hipStream_t s = c->stream;
+    int rc = kernel_error(c);
+    if (rc) return rc;
+    Globals g0;
+    HIPCHK(hipMemcpy(&g0, b.g, sizeof(g0), hipMemcpyDeviceToHost));
+    // everything the coming step does before its walk
+    launch_begin_step(b, s);
+    launch_keys_sort_permute(b, s);
+    launch_tree(b, s);
+    launch_targets(b, s);
+    HIPCHK(hipStreamSynchronize(s));
+    rc = kernel_error(c);
+    if (rc) return rc;
+    const uint32_t cus = (uint32_t)device_cus();
+    const uint32_t pms[] = {0, 4, 8, 12, 16, 24, 32, 48, 64};
+    const uint32_t bpc[] = {4, 8, 16};
+    const uint32_t keep_pm = b.pair_max, keep_wb = b.walk_blocks;
+    double best = 1e30;
+    uint32_t best_pm = keep_pm, best_wb = keep_wb;
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    for (uint32_t wb : bpc) {
+        for (uint32_t pm : pms) {
+            if (!b.precision && pm != pms[0]) continue;  // the float64 walk has no pair mode
+            b.pair_max = pm;
+            b.walk_blocks = wb * cus;
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                launch_walk_init(b, s);
+                launch_walk_reset(b, s);
+                HIPCHK(hipEventRecord(e0, s));
+                launch_walk(b, s);
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < ms_best) ms_best = ms;
+            }
+            if (ms_best < best) {
+                best = ms_best;
+                best_pm = pm;
+                best_wb = b.walk_blocks;
+            }
+        }
+    }
+    rc = kernel_error(c);
+    // the step was not taken: step counter and work counters as before
+    Globals g1;
+    HIPCHK(hipMemcpy(&g1, b.g, sizeof(g1), hipMemcpyDeviceToHost));
+    g1.t = g0.t;
+    g1.n_cell = g0.n_cell;
+    g1.n_leaf = g0.n_leaf;
+    g1.n_open = g0.n_open;
+    g1.n_pops = g0.n_pops;
+    g1.n_tgt_total = g0.n_tgt_total;
+    g1.n_steps = g0.n_steps;
+    HIPCHK(hipMemcpy(b.g, &g1, sizeof(g1), hipMemcpyHostToDevice));
+    b.pair_max = rc ? keep_pm : best_pm;
+    b.walk_blocks = rc ? keep_wb : best_wb;
+    c->tuned_ms = rc ? 0.0 : best;
+    destroy_graphs(c);
+    if (rc) return rc;
+    return potg_set_tuning(c, nullptr, out);
 }
 
 int potg_stage_times(potg_ctx* c, int32_t enable, double* ms5)

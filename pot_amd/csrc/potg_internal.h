@@ -31,6 +31,7 @@ struct Bufs {
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
     uint32_t pair_max; // walk: ranges with at most this many targets go pair-parallel
+    uint32_t walk_blocks; // blocks of the persistent walk grid
     InstParams* P;
     InstState* S;
     Globals* g;
@@ -91,6 +92,8 @@ void launch_targets(Bufs& b, hipStream_t s);
 void launch_chi_b(Bufs& b, hipStream_t s);
 void launch_walk_init(Bufs& b, hipStream_t s);  // grain term of each target
 void launch_walk(Bufs& b, hipStream_t s);
+void launch_walk_reset(Bufs& b, hipStream_t s);  // rewind the walk's tile counter
+int device_cus();
 void launch_push(Bufs& b, hipStream_t s);
 // compute_epe nonzero: fold the potential-energy sums of launch_epe
 void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s);

@@ -637,21 +637,23 @@ __global__ void k_grain_init(const InstParams* __restrict__ P, const InstState* 
 
 __global__ void k_reset_tiles(uint32_t* ctr) { *ctr = 0; }
 
-int walk_grid()
+}  // namespace
+
+int device_cus()
 {
-    static int grid = 0;
-    if (!grid) {
-        int dev = 0, cus = 256;
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) {
             hipDeviceProp_t pr;
             if (hipGetDeviceProperties(&pr, dev) == hipSuccess) cus = pr.multiProcessorCount;
         }
-        grid = cus * 8;
     }
-    return grid;
+    return cus;
 }
 
-}  // namespace
+void launch_walk_reset(Bufs& b, hipStream_t s) { k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr); }
 
 static WalkIO step_io(Bufs& b)
 {
@@ -677,9 +679,9 @@ void launch_walk(Bufs& b, hipStream_t s)
 {
     WalkIO io = step_io(b);
     if (b.precision)
-        k_walk32<MODE_STEP><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
+        k_walk32<MODE_STEP><<<b.walk_blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
     else
-        k_walk64<MODE_STEP, KIND_FIELD><<<walk_grid(), BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+        k_walk64<MODE_STEP, KIND_FIELD><<<b.walk_blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
 }
 
 void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
@@ -697,7 +699,7 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     io.dip32 = b.dip32;
     k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr2);
     const uint32_t nb = (n + BT - 1) / BT ? (n + BT - 1) / BT : 1;
-    const int blocks = (int)nb < walk_grid() ? (int)nb : walk_grid();
+    const int blocks = (int)nb < (int)b.walk_blocks ? (int)nb : (int)b.walk_blocks;
     if (kind == 1) {
         k_grain_init<MODE_POINTS, KIND_PHI><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
         k_walk64<MODE_POINTS, KIND_PHI><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
@@ -756,7 +758,7 @@ void launch_epe(Bufs& b, hipStream_t s)
     io.dip64 = b.dip64;
     k_reset_tiles<<<1, 1, 0, s>>>(&b.g->tile_ctr2);
     const uint32_t nb = (b.n_total + BT - 1) / BT;
-    const int blocks = (int)nb < walk_grid() ? (int)nb : walk_grid();
+    const int blocks = (int)nb < (int)b.walk_blocks ? (int)nb : (int)b.walk_blocks;
     k_grain_init<MODE_EPE, KIND_EPE><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
     k_walk64<MODE_EPE, KIND_EPE><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
     k_epe_partial<<<dim3(EPE_BLOCKS, b.n_inst), BT, 0, s>>>(b.P, b.st, b.phi, b.epe_part);

@@ -26,7 +26,8 @@ ERRORS = {-1: "POTG_ERR_ARG", -2: "POTG_ERR_HIP", -3: "POTG_ERR_OUT_OF_BOX",
 EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "potg_download",
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
-           "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles"]
+           "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles", "potg_tune",
+           "potg_set_tuning"]
 
 
 class Config(C.Structure):
@@ -55,6 +56,10 @@ class Stats(C.Structure):
                 ("n_leaf", C.c_uint64), ("n_open", C.c_uint64), ("n_nodes", C.c_uint64),
                 ("n_particles", C.c_uint64), ("max_depth", C.c_uint64),
                 ("n_range_loads", C.c_uint64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pair_max", C.c_uint32), ("walk_blocks", C.c_uint32), ("walk_ms", C.c_double)]
 
 
 class PotgError(RuntimeError):
@@ -93,6 +98,8 @@ def lib():
     L.potg_stage_times.argtypes = [P, C.c_int32, dp]
     L.potg_sync.argtypes = [P]
     L.potg_tree_canon_dipoles.argtypes = [P, C.c_uint32, P, C.c_int64, C.POINTER(C.c_int64)]
+    L.potg_tune.argtypes = [P, C.POINTER(Tuning)]
+    L.potg_set_tuning.argtypes = [P, C.POINTER(Tuning), C.POINTER(Tuning)]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
@@ -231,6 +238,21 @@ class Context:
         E = np.zeros((len(idx), 3))
         _check(self.L.potg_direct_sum(self.h, inst, _up(idx), len(idx), _dp(E)))
         return E
+
+    def tune(self):
+        t = Tuning()
+        _check(self.L.potg_tune(self.h, C.byref(t)))
+        return {"pair_max": t.pair_max, "walk_blocks": t.walk_blocks, "walk_ms": t.walk_ms}
+
+    def tuning(self):
+        t = Tuning()
+        _check(self.L.potg_set_tuning(self.h, None, C.byref(t)))
+        return {"pair_max": t.pair_max, "walk_blocks": t.walk_blocks, "walk_ms": t.walk_ms}
+
+    def set_tuning(self, pair_max, walk_blocks):
+        t = Tuning()
+        t.pair_max, t.walk_blocks = pair_max, walk_blocks
+        _check(self.L.potg_set_tuning(self.h, C.byref(t), None))
 
     def stats(self):
         s = Stats()

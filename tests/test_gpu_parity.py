@@ -174,6 +174,57 @@ def test_step_potential_energies_match_oracle(case):
                 assert abs(gr["epe"][s] - want) <= 1e-11 * abs(want), (step, s, gr["epe"][s], want)
 
 
+def test_tuning_takes_no_step_and_is_reproducible():
+    """potg_tune times the coming step's walk without taking the step: a
+    context that tunes and then steps matches one that was given the same
+    launch parameters directly, bit for bit, and the step counter and work
+    counters are untouched by the tuning."""
+    case = big_case(20001, seed=23)
+    parts = oracle_sim(case).particles()
+    for prec in (G.POTG_MIXED, G.POTG_FP64):
+        a = gpu_ctx(case, parts, precision=prec, seed=3)
+        b = gpu_ctx(case, parts, precision=prec, seed=3)
+        a.step()
+        b.step()
+        before = a.stats()
+        t = a.tune()
+        assert 0 <= t["pair_max"] <= 64 and t["walk_blocks"] > 0 and t["walk_ms"] > 0
+        after = a.stats()
+        for k in ("steps", "n_cell", "n_leaf", "n_open", "n_range_loads"):
+            assert after[k] == before[k], k
+        b.set_tuning(t["pair_max"], t["walk_blocks"])
+        for _ in range(3):
+            a.step()
+            b.step()
+        pa, _ = a.download(0)
+        pb, _ = b.download(0)
+        for f in ("x", "y", "z", "vx", "vy", "vz", "species"):
+            assert np.array_equal(pa[f], pb[f]), (prec, f)
+        assert a.stats()["steps"] == 4
+
+
+@pytest.mark.parametrize("pair_max", [0, 1, 7, 64])
+def test_mixed_walk_pair_thresholds_agree(pair_max):
+    """Every pair threshold gives the same interaction set (counts equal to
+    the float64 walk's) and the same field to float32 accuracy."""
+    case = BIG[0]
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    ctx64 = gpu_ctx(case, parts)
+    ctx64.build_tree()
+    ctx = gpu_ctx(case, parts, precision=G.POTG_MIXED)
+    ctx.set_tuning(pair_max, 2048)
+    ctx.build_tree()
+    idx = np.arange(1, len(parts), 3, dtype=np.uint32)
+    xyz = np.stack([parts["x"][idx], parts["y"][idx], parts["z"][idx]], 1)
+    E64, c64 = ctx64.field_at(0, xyz, exclude=idx, counts=True)
+    E32, c32 = ctx.field_at(0, xyz, exclude=idx, counts=True)
+    same = np.all(c32 == c64, axis=1)
+    assert same.mean() > 0.99  # float32 MAC decisions may flip at the threshold
+    rel = np.linalg.norm(E32 - E64, axis=1) / np.linalg.norm(E64, axis=1)
+    assert np.median(rel) < 1e-5
+
+
 def test_graph_replay_equals_eager_steps():
     case = big_case(20001, seed=21)
     parts = oracle_sim(case).particles()
