@@ -347,6 +347,8 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.wid_int, ci) | alloc_n(c, &b.wid_leaf, nt);
     rc |= alloc_n(c, &b.par_int, ci) | alloc_n(c, &b.par_leaf, nt) | alloc_n(c, &b.arrive, ci);
     rc |= alloc_n(c, &b.size2, ci);
+    rc |= alloc_n(c, &b.lvl_count, MAXD + 1) | alloc_n(c, &b.lvl_start, MAXD + 1);
+    rc |= alloc_n(c, &b.lvl_fill, MAXD + 1) | alloc_n(c, &b.by_level, ci);
     rc |= alloc_n(c, &b.mcx, ci) | alloc_n(c, &b.mcy, ci) | alloc_n(c, &b.mcz, ci);
     rc |= alloc_n(c, &b.mq, ci) | alloc_n(c, &b.maq, ci);
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);

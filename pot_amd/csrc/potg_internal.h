@@ -54,6 +54,8 @@ struct Bufs {
     uint32_t* par_int;
     uint32_t* par_leaf;
     uint32_t* arrive;
+    uint32_t *lvl_count, *lvl_start, *lvl_fill;  // internal nodes per depth
+    uint32_t* by_level;  // internal node ids grouped by depth
     double* size2;
     double *mcx, *mcy, *mcz;
     int32_t* mq;

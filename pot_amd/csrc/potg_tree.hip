@@ -446,26 +446,93 @@ __global__ void k_leaf_records(PState st, const InstParams* __restrict__ P,
     if (dip32) dip32[w] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// moments of the internal nodes at one depth, children folded in the
-// reference order (compute_node_values / update_node_moments,
-// pot.c:1290-1402): charge, |charge|, centre of |charge| and, for DIPOLES
-// instances, the dipole moment about that centre; the deeper levels were
-// finished by earlier launches
-__global__ void k_level_moments(int L, PState st, const InstParams* __restrict__ P,
-                                const InstState* __restrict__ S, const Globals* __restrict__ g,
-                                uint32_t cap, const int32_t* __restrict__ level,
-                                const uint32_t* __restrict__ owner,
-                                const uint32_t* __restrict__ slots,
-                                const uint32_t* __restrict__ ccount,
-                                const uint32_t* __restrict__ cfirst,
-                                const uint32_t* __restrict__ wid_int,
-                                const double* __restrict__ size2, NodeMoments m,
-                                NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
-                                uint32_t n_rec)
+// Internal nodes bucketed by depth, so that each level's moment launch
+// visits only that level's nodes: counts, offsets, then a scatter in which
+// each block reserves one slice per level.  Order within a level is free
+// (the nodes of one level are independent).
+__global__ void k_level_clear(uint32_t* __restrict__ cnt)
 {
-    if ((uint32_t)L + 1 > g->max_depth) return;  // no internal node this deep
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_int_of(g, cap) || level[k] != L) return;
+    if (threadIdx.x < MAXD) cnt[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(BT) void k_level_count(const int32_t* __restrict__ level,
+                                                    const Globals* __restrict__ g, uint32_t cap,
+                                                    uint32_t* __restrict__ cnt)
+{
+    __shared__ uint32_t h[MAXD];
+    if (threadIdx.x < MAXD) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t nint = n_int_of(g, cap);
+    for (uint32_t k = blockIdx.x * BT + threadIdx.x; k < nint; k += gridDim.x * BT) {
+        const int32_t L = level[k];
+        if (L >= 0 && L < MAXD) atomicAdd(&h[L], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < MAXD && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_level_offsets(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
+                                uint32_t* __restrict__ fill)
+{
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (int L = 0; L < MAXD; L++) {
+            start[L] = a;
+            a += cnt[L];
+        }
+        start[MAXD] = a;
+    }
+    if (threadIdx.x < MAXD) fill[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(BT) void k_level_scatter(const int32_t* __restrict__ level,
+                                                      const Globals* __restrict__ g, uint32_t cap,
+                                                      const uint32_t* __restrict__ start,
+                                                      uint32_t* __restrict__ fill,
+                                                      uint32_t* __restrict__ by_level)
+{
+    __shared__ uint32_t h[MAXD], base[MAXD];
+    const uint32_t nint = n_int_of(g, cap);
+    for (uint32_t k0 = blockIdx.x * BT; k0 < nint; k0 += gridDim.x * BT) {
+        if (threadIdx.x < MAXD) h[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t k = k0 + threadIdx.x;
+        int32_t L = -1;
+        uint32_t rank = 0;
+        if (k < nint) {
+            L = level[k];
+            if (L >= 0 && L < MAXD) rank = atomicAdd(&h[L], 1u);
+            else L = -1;
+        }
+        __syncthreads();
+        if (threadIdx.x < MAXD && h[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
+        __syncthreads();
+        if (L >= 0) {
+            const uint32_t pos = start[L] + base[L] + rank;
+            if (pos < cap) by_level[pos] = k;
+        }
+        __syncthreads();
+    }
+}
+
+// the moments of internal node k, its children folded in the reference
+// order (compute_node_values / update_node_moments, pot.c:1290-1402):
+// charge, |charge|, centre of |charge| and, for DIPOLES instances, the
+// dipole moment about that centre
+__device__ void level_node_moments(uint32_t k, PState st, const InstParams* __restrict__ P,
+                                   const InstState* __restrict__ S, uint32_t cap,
+                                   const int32_t* __restrict__ level,
+                                   const uint32_t* __restrict__ owner,
+                                   const uint32_t* __restrict__ slots,
+                                   const uint32_t* __restrict__ ccount,
+                                   const uint32_t* __restrict__ cfirst,
+                                   const uint32_t* __restrict__ wid_int,
+                                   const double* __restrict__ size2, NodeMoments m,
+                                   NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
+                                   uint32_t n_rec)
+{
+    if (k >= cap) return;
     const uint32_t inst = st.inst[owner[k]];
     const int32_t gq = S[inst].grain_q;
     const bool dip = P[inst].dipoles && m.px;
@@ -551,6 +618,29 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
                                  (float)(pz * P[inst].invR), 0.f);
 }
 
+// the moments of the internal nodes at depth L; the deeper levels were
+// finished by earlier launches
+__global__ void k_level_moments(int L, PState st, const InstParams* __restrict__ P,
+                                const InstState* __restrict__ S, const Globals* __restrict__ g,
+                                uint32_t cap, const int32_t* __restrict__ level,
+                                const uint32_t* __restrict__ owner,
+                                const uint32_t* __restrict__ slots,
+                                const uint32_t* __restrict__ ccount,
+                                const uint32_t* __restrict__ cfirst,
+                                const uint32_t* __restrict__ wid_int,
+                                const double* __restrict__ size2, NodeMoments m,
+                                NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
+                                uint32_t n_rec, const uint32_t* __restrict__ lstart,
+                                const uint32_t* __restrict__ by_level)
+{
+    if ((uint32_t)L + 1 > g->max_depth) return;  // no internal node this deep
+    const uint32_t lo = lstart[L], hi = lstart[L + 1];
+    for (uint32_t idx = lo + blockIdx.x * blockDim.x + threadIdx.x; idx < hi;
+         idx += gridDim.x * blockDim.x)
+        level_node_moments(by_level[idx], st, P, S, cap, level, owner, slots, ccount, cfirst,
+                           wid_int, size2, m, r64, r32, n_rec);
+}
+
 }  // namespace
 
 void launch_begin_step(Bufs& b, hipStream_t s)
@@ -595,12 +685,17 @@ void launch_tree(Bufs& b, hipStream_t s)
         k_leaf_records<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.n_rec, b.wid_leaf, b.rec64,
                                               r32, b.dip64, d32);
         NodeMoments m{b.mcx, b.mcy, b.mcz, b.mpx, b.mpy, b.mpz, b.mq, b.maq, b.dip64, d32};
+        const uint32_t cap = b.cap_int;
+        const uint32_t gb = grid(cap) < 2048u ? grid(cap) : 2048u;
+        k_level_clear<<<1, 64, 0, s>>>(b.lvl_count);
+        k_level_count<<<gb, BT, 0, s>>>(b.level, b.g, cap, b.lvl_count);
+        k_level_offsets<<<1, 64, 0, s>>>(b.lvl_count, b.lvl_start, b.lvl_fill);
+        k_level_scatter<<<gb, BT, 0, s>>>(b.level, b.g, cap, b.lvl_start, b.lvl_fill, b.by_level);
         // deepest level first; launches past the tree's depth return at once
         for (int L = MAXD - 1; L >= 0; L--)
-            k_level_moments<<<grid(b.cap_int), BT, 0, s>>>(L, b.st, b.P, b.S, b.g, b.cap_int,
-                                                           b.level, b.owner, b.slots, b.ccount,
-                                                           b.cfirst, b.wid_int, b.size2, m,
-                                                           b.rec64, r32, b.n_rec);
+            k_level_moments<<<gb, BT, 0, s>>>(L, b.st, b.P, b.S, b.g, cap, b.level, b.owner,
+                                              b.slots, b.ccount, b.cfirst, b.wid_int, b.size2, m,
+                                              b.rec64, r32, b.n_rec, b.lvl_start, b.by_level);
     }
 }
 
