@@ -46,7 +46,7 @@ build_pot() {  # name theta bx by bz [extra -D...]
         MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
             -DSPHERICAL -DSCEPTIC_REINJECTION -DOAP="$theta" \
             -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
-            -DRANDOMNESS_SOURCE='"seed.bin"' "$@" -o "$OUT/$name" -lm
+            -DRANDOMNESS_SOURCE="${SEED_SOURCE:-\"seed.bin\"}" "$@" -o "$OUT/$name" -lm
 }
 
 build_sph() {  # name shape theta bx by bz
@@ -71,6 +71,11 @@ build_pot pot_sphere_dip 0.5 0.0 0.0 20.0 -DDIPOLES
 # benchmark workloads (pot_amd/workloads.py): sphere_2m_bz (B along z,
 # r_ge = lambda_D / 2) and prolate_4m_tilted_b (B 30 degrees off the axis)
 build_pot pot_sphere_2m_bz 0.5 0.0 0.0 3.72363
+# the same for timing under mpiexec: every rank seeds from /dev/urandom, as
+# the shipped pot.c does.  (With one seed file every rank draws the same
+# reinjection stream, so ranks reinject particles onto the same points and
+# the next tree build never separates them.)
+SEED_SOURCE='"/dev/urandom"' build_pot pot_sphere_2m_bz_mpi 0.5 0.0 0.0 3.72363
 # spheroidal grain (pot-spheroid.c; prolate_4m_tilted_b has B 30 degrees off the x axis)
 build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
 build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0

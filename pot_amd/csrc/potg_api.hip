@@ -310,6 +310,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     // targets go pair-parallel; 8 blocks of 4 waves per CU
     b.pair_max = 16;
     b.walk_blocks = 8 * device_cus();
+    b.group = 64;
 
     c->hp.resize(n_inst);
     uint32_t off = 0, tb = 0;
@@ -347,11 +348,13 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.wid_int, ci) | alloc_n(c, &b.wid_leaf, nt);
     rc |= alloc_n(c, &b.par_int, ci) | alloc_n(c, &b.par_leaf, nt) | alloc_n(c, &b.arrive, ci);
     rc |= alloc_n(c, &b.size2, ci);
-    rc |= alloc_n(c, &b.lvl_count, MAXD + 1) | alloc_n(c, &b.lvl_start, MAXD + 1);
-    rc |= alloc_n(c, &b.lvl_fill, MAXD + 1) | alloc_n(c, &b.by_level, ci);
+    rc |= alloc_n(c, &b.lvl_hist, (size_t)MAXD * LVL_BLOCKS);
+    rc |= alloc_n(c, &b.lvl_off, (size_t)MAXD * LVL_BLOCKS);
+    rc |= alloc_n(c, &b.lvl_start, MAXD + 1) | alloc_n(c, &b.by_level, ci);
     rc |= alloc_n(c, &b.mcx, ci) | alloc_n(c, &b.mcy, ci) | alloc_n(c, &b.mcz, ci);
     rc |= alloc_n(c, &b.mq, ci) | alloc_n(c, &b.maq, ci);
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
+
     rc |= alloc_n(c, &b.rec64, b.n_rec);
     if (precision == POTG_MIXED) rc |= alloc_n(c, &b.rec32, b.n_rec);
     bool any_dip = false;
@@ -762,16 +765,21 @@ int potg_set_tuning(potg_ctx* c, const potg_tuning* set, potg_tuning* get)
 {
     if (!c) return set_err("potg_set_tuning: null context"), POTG_ERR_ARG;
     if (set) {
-        if (set->pair_max > 64 || set->walk_blocks == 0 || set->walk_blocks > 65536)
-            return set_err("potg_set_tuning: pair_max must be 0..64, walk_blocks 1..65536"),
+        if (set->pair_max > 64 || set->walk_blocks == 0 || set->walk_blocks > 65536 ||
+            (set->group != 8 && set->group != 16 && set->group != 64))
+            return set_err("potg_set_tuning: pair_max must be 0..64, walk_blocks 1..65536, "
+                           "group 8, 16 or 64"),
                    POTG_ERR_ARG;
         c->b.pair_max = set->pair_max;
         c->b.walk_blocks = set->walk_blocks;
+        c->b.group = set->group;
         destroy_graphs(c);  // captured steps hold the old launch parameters
     }
     if (get) {
         get->pair_max = c->b.pair_max;
         get->walk_blocks = c->b.walk_blocks;
+        get->group = c->b.group;
+        get->pad = 0;
         get->walk_ms = c->tuned_ms;
     }
     return POTG_OK;
@@ -796,16 +804,19 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     rc = kernel_error(c);
     if (rc) return rc;
     const uint32_t cus = (uint32_t)device_cus();
-    const uint32_t pms[] = {0, 4, 8, 12, 16, 24, 32, 48, 64};
+    // candidates: (group, pair threshold); the float64 walk has neither
+    const uint32_t cand[][2] = {{8, 0},  {16, 0}, {64, 0},  {64, 4},  {64, 8},
+                                {64, 16}, {64, 32}, {64, 64}};
     const uint32_t bpc[] = {4, 8, 16};
-    const uint32_t keep_pm = b.pair_max, keep_wb = b.walk_blocks;
+    const uint32_t keep_pm = b.pair_max, keep_wb = b.walk_blocks, keep_gr = b.group;
     double best = 1e30;
-    uint32_t best_pm = keep_pm, best_wb = keep_wb;
+    uint32_t best_pm = keep_pm, best_wb = keep_wb, best_gr = keep_gr;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     for (uint32_t wb : bpc) {
-        for (uint32_t pm : pms) {
-            if (!b.precision && pm != pms[0]) continue;  // the float64 walk has no pair mode
-            b.pair_max = pm;
+        for (const auto& cd : cand) {
+            if (!b.precision && (cd[0] != 64 || cd[1] != 0)) continue;
+            b.group = cd[0];
+            b.pair_max = cd[1];
             b.walk_blocks = wb * cus;
             float ms_best = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
@@ -821,8 +832,9 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
             }
             if (ms_best < best) {
                 best = ms_best;
-                best_pm = pm;
+                best_pm = b.pair_max;
                 best_wb = b.walk_blocks;
+                best_gr = b.group;
             }
         }
     }
@@ -840,6 +852,7 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     HIPCHK(hipMemcpy(b.g, &g1, sizeof(g1), hipMemcpyHostToDevice));
     b.pair_max = rc ? keep_pm : best_pm;
     b.walk_blocks = rc ? keep_wb : best_wb;
+    b.group = rc ? keep_gr : best_gr;
     c->tuned_ms = rc ? 0.0 : best;
     destroy_graphs(c);
     if (rc) return rc;

@@ -25,6 +25,11 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;
 constexpr uint32_t LEAF_BIT = 0x80000000u; // child-slot code of a leaf
 constexpr int WAVE = 64;
 constexpr int EPE_BLOCKS = 64;    // partial sums per instance of the PE reduction
+// tree moments: blocks of the depth counting sort, and the band of depths
+// given a launch per level (the others are done by one block)
+constexpr uint32_t LVL_BLOCKS = 256;
+constexpr int LVL_WIDE_LO = 12;   // depths >= this: one block
+constexpr int LVL_WIDE_HI = 3;    // depths <= this: one block
 
 // error codes raised by kernels (mirrors POTG_ERR_*)
 constexpr int KERR_OUT_OF_BOX = 1;

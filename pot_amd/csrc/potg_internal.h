@@ -32,6 +32,7 @@ struct Bufs {
     int key_bits;      // bits of the key that are sorted
     uint32_t pair_max; // walk: ranges with at most this many targets go pair-parallel
     uint32_t walk_blocks; // blocks of the persistent walk grid
+    uint32_t group;    // float32 walk: targets per wavefront (8, 16 or 64)
     InstParams* P;
     InstState* S;
     Globals* g;
@@ -54,7 +55,8 @@ struct Bufs {
     uint32_t* par_int;
     uint32_t* par_leaf;
     uint32_t* arrive;
-    uint32_t *lvl_count, *lvl_start, *lvl_fill;  // internal nodes per depth
+    uint32_t *lvl_hist, *lvl_off;  // per-block depth histograms and their scan
+    uint32_t* lvl_start;           // first slot of each depth in by_level
     uint32_t* by_level;  // internal node ids grouped by depth
     double* size2;
     double *mcx, *mcy, *mcz;

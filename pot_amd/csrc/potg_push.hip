@@ -384,15 +384,6 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
         st.vy[i] = vy;
         st.vz[i] = vz;
     }
-    // pushed count: one atomic per instance present in the wavefront
-    uint64_t pending = ballot(valid);
-    while (pending) {
-        const int lead = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t gi = __shfl(inst, lead, WAVE);
-        const uint64_t grp = ballot(valid && inst == gi);
-        pending &= ~grp;
-        if ((int)__lane_id() == lead) atomicAdd(&S[gi].n_pushed, (uint32_t)__popcll(grp));
-    }
 }
 
 // Reinjections queued by k_push: (slot << 1) | absorbed.  An absorbed
@@ -479,7 +470,9 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
         s.out_q = s.acc_q;
         s.out_e = s.acc_e;
         s.out_i = s.acc_i;
-        s.out_pushed = s.n_pushed;
+        // every index from the first stepped one to the last was pushed
+        // (pot.c:2436-2440)
+        s.out_pushed = s.start == NONE ? 0u : p.N - s.start;
         s.out_reinj_abs = s.n_reinj_abs;
         s.out_reinj_out = s.n_reinj_out;
     }

@@ -192,13 +192,20 @@ __global__ void k_lcp(const uint64_t* __restrict__ key, PState st, const InstPar
         }
     }
     if (i < n_tree) lcp[i] = v;
-    // deepest internal level + 1, one atomic per wavefront
+    // deepest internal level + 1: reduced over the block, and the atomic
+    // only where it raises the value (it rarely does after the first step)
+    __shared__ uint32_t wmax[BT / WAVE];
     uint32_t m = (uint32_t)(v + 1);
     for (int d = 1; d < WAVE; d <<= 1) {
         const uint32_t o = __shfl_xor(m, d, WAVE);
         m = o > m ? o : m;
     }
-    if (__lane_id() == 0) atomicMax(&g->max_depth, m);
+    if (__lane_id() == 0) wmax[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BT / WAVE; w++) m = wmax[w] > m ? wmax[w] : m;
+        if (m > *(volatile const uint32_t*)&g->max_depth) atomicMax(&g->max_depth, m);
+    }
 }
 
 __global__ void k_counts(const int32_t* __restrict__ lcp, uint32_t n_tree, uint32_t* __restrict__ cnt)
@@ -446,73 +453,99 @@ __global__ void k_leaf_records(PState st, const InstParams* __restrict__ P,
     if (dip32) dip32[w] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Internal nodes bucketed by depth, so that each level's moment launch
-// visits only that level's nodes: counts, offsets, then a scatter in which
-// each block reserves one slice per level.  Order within a level is free
-// (the nodes of one level are independent).
-__global__ void k_level_clear(uint32_t* __restrict__ cnt)
+// Internal nodes bucketed by depth, so that each level's moment pass visits
+// only that level's nodes: a counting sort without atomics.  Block b owns a
+// contiguous chunk of the node ids; per-block level histograms are scanned
+// level-major, then each block writes its nodes at its offsets.  Order
+// within a level is free (the nodes of one level are independent).
+__device__ inline void level_chunk(const Globals* g, uint32_t cap, uint32_t& lo, uint32_t& hi)
 {
-    if (threadIdx.x < MAXD) cnt[threadIdx.x] = 0;
+    const uint32_t nint = n_int_of(g, cap);
+    const uint32_t chunk = (nint + LVL_BLOCKS - 1) / LVL_BLOCKS;
+    lo = blockIdx.x * chunk;
+    hi = lo + chunk < nint ? lo + chunk : nint;
+    if (lo > hi) lo = hi;
 }
 
 __global__ __launch_bounds__(BT) void k_level_count(const int32_t* __restrict__ level,
                                                     const Globals* __restrict__ g, uint32_t cap,
-                                                    uint32_t* __restrict__ cnt)
+                                                    uint32_t* __restrict__ hist)
 {
     __shared__ uint32_t h[MAXD];
     if (threadIdx.x < MAXD) h[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t nint = n_int_of(g, cap);
-    for (uint32_t k = blockIdx.x * BT + threadIdx.x; k < nint; k += gridDim.x * BT) {
+    uint32_t lo, hi;
+    level_chunk(g, cap, lo, hi);
+    for (uint32_t k = lo + threadIdx.x; k < hi; k += BT) {
         const int32_t L = level[k];
         if (L >= 0 && L < MAXD) atomicAdd(&h[L], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < MAXD && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < MAXD) hist[threadIdx.x * LVL_BLOCKS + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ void k_level_offsets(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
-                                uint32_t* __restrict__ fill)
+// exclusive scan of the MAXD x LVL_BLOCKS histogram in level-major order;
+// lstart[L] = first slot of level L, lstart[MAXD] = all internal nodes
+__global__ __launch_bounds__(1024) void k_level_offsets(const uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ off,
+                                                        uint32_t* __restrict__ lstart)
 {
-    if (threadIdx.x == 0) {
-        uint32_t a = 0;
-        for (int L = 0; L < MAXD; L++) {
-            start[L] = a;
-            a += cnt[L];
-        }
-        start[MAXD] = a;
+    constexpr uint32_t N = MAXD * LVL_BLOCKS;
+    constexpr uint32_t PER = (N + 1023) / 1024;
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    uint32_t sum = 0;
+    for (uint32_t j = 0; j < PER; j++) {
+        const uint32_t i = t * PER + j;
+        if (i < N) sum += hist[i];
     }
-    if (threadIdx.x < MAXD) fill[threadIdx.x] = 0;
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;  // exclusive prefix of this thread's run
+    for (uint32_t j = 0; j < PER; j++) {
+        const uint32_t i = t * PER + j;
+        if (i < N) {
+            off[i] = run;
+            if (i % LVL_BLOCKS == 0) lstart[i / LVL_BLOCKS] = run;
+            run += hist[i];
+        }
+    }
+    if (t == 1023) lstart[MAXD] = part[1023];
 }
 
 __global__ __launch_bounds__(BT) void k_level_scatter(const int32_t* __restrict__ level,
                                                       const Globals* __restrict__ g, uint32_t cap,
-                                                      const uint32_t* __restrict__ start,
-                                                      uint32_t* __restrict__ fill,
+                                                      const uint32_t* __restrict__ off,
                                                       uint32_t* __restrict__ by_level)
 {
     __shared__ uint32_t h[MAXD], base[MAXD];
-    const uint32_t nint = n_int_of(g, cap);
-    for (uint32_t k0 = blockIdx.x * BT; k0 < nint; k0 += gridDim.x * BT) {
+    uint32_t lo, hi;
+    level_chunk(g, cap, lo, hi);
+    if (threadIdx.x < MAXD) base[threadIdx.x] = off[threadIdx.x * LVL_BLOCKS + blockIdx.x];
+    for (uint32_t k0 = lo; k0 < hi; k0 += BT) {
         if (threadIdx.x < MAXD) h[threadIdx.x] = 0;
         __syncthreads();
         const uint32_t k = k0 + threadIdx.x;
         int32_t L = -1;
         uint32_t rank = 0;
-        if (k < nint) {
+        if (k < hi) {
             L = level[k];
             if (L >= 0 && L < MAXD) rank = atomicAdd(&h[L], 1u);
             else L = -1;
         }
         __syncthreads();
-        if (threadIdx.x < MAXD && h[threadIdx.x])
-            base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
-        __syncthreads();
         if (L >= 0) {
-            const uint32_t pos = start[L] + base[L] + rank;
+            const uint32_t pos = base[L] + rank;
             if (pos < cap) by_level[pos] = k;
         }
         __syncthreads();
+        if (threadIdx.x < MAXD) base[threadIdx.x] += h[threadIdx.x];
     }
 }
 
@@ -641,6 +674,30 @@ __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__
                            wid_int, size2, m, r64, r32, n_rec);
 }
 
+// levels Lhi down to Llo by one block, a barrier between levels: for the
+// thinly populated deep and shallow levels, where a launch per level would
+// cost more than the work
+__global__ __launch_bounds__(1024) void k_level_moments_block(
+    int Lhi, int Llo, PState st, const InstParams* __restrict__ P, const InstState* __restrict__ S,
+    const Globals* __restrict__ g, uint32_t cap, const int32_t* __restrict__ level,
+    const uint32_t* __restrict__ owner, const uint32_t* __restrict__ slots,
+    const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ cfirst,
+    const uint32_t* __restrict__ wid_int, const double* __restrict__ size2, NodeMoments m,
+    NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32, uint32_t n_rec,
+    const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ by_level)
+{
+    const uint32_t depth = g->max_depth;
+    for (int L = Lhi; L >= Llo; L--) {
+        if ((uint32_t)L + 1 > depth) continue;  // uniform over the block
+        const uint32_t lo = lstart[L], hi = lstart[L + 1];
+        for (uint32_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x)
+            level_node_moments(by_level[idx], st, P, S, cap, level, owner, slots, ccount, cfirst,
+                               wid_int, size2, m, r64, r32, n_rec);
+        // the next level up reads what this one wrote
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 void launch_begin_step(Bufs& b, hipStream_t s)
@@ -687,15 +744,25 @@ void launch_tree(Bufs& b, hipStream_t s)
         NodeMoments m{b.mcx, b.mcy, b.mcz, b.mpx, b.mpy, b.mpz, b.mq, b.maq, b.dip64, d32};
         const uint32_t cap = b.cap_int;
         const uint32_t gb = grid(cap) < 2048u ? grid(cap) : 2048u;
-        k_level_clear<<<1, 64, 0, s>>>(b.lvl_count);
-        k_level_count<<<gb, BT, 0, s>>>(b.level, b.g, cap, b.lvl_count);
-        k_level_offsets<<<1, 64, 0, s>>>(b.lvl_count, b.lvl_start, b.lvl_fill);
-        k_level_scatter<<<gb, BT, 0, s>>>(b.level, b.g, cap, b.lvl_start, b.lvl_fill, b.by_level);
-        // deepest level first; launches past the tree's depth return at once
-        for (int L = MAXD - 1; L >= 0; L--)
+        k_level_count<<<LVL_BLOCKS, BT, 0, s>>>(b.level, b.g, cap, b.lvl_hist);
+        k_level_offsets<<<1, 1024, 0, s>>>(b.lvl_hist, b.lvl_off, b.lvl_start);
+        k_level_scatter<<<LVL_BLOCKS, BT, 0, s>>>(b.level, b.g, cap, b.lvl_off, b.by_level);
+        // deepest level first.  Levels below LVL_WIDE_LO hold few nodes in
+        // any plasma (their cells are a few mean spacings wide or less), as
+        // do those above LVL_WIDE_HI: one block each; the levels between
+        // get a launch of their own
+        k_level_moments_block<<<1, 1024, 0, s>>>(MAXD - 1, LVL_WIDE_LO, b.st, b.P, b.S, b.g, cap,
+                                                 b.level, b.owner, b.slots, b.ccount, b.cfirst,
+                                                 b.wid_int, b.size2, m, b.rec64, r32, b.n_rec,
+                                                 b.lvl_start, b.by_level);
+        for (int L = LVL_WIDE_LO - 1; L > LVL_WIDE_HI; L--)
             k_level_moments<<<gb, BT, 0, s>>>(L, b.st, b.P, b.S, b.g, cap, b.level, b.owner,
                                               b.slots, b.ccount, b.cfirst, b.wid_int, b.size2, m,
                                               b.rec64, r32, b.n_rec, b.lvl_start, b.by_level);
+        k_level_moments_block<<<1, 1024, 0, s>>>(LVL_WIDE_HI, 0, b.st, b.P, b.S, b.g, cap,
+                                                 b.level, b.owner, b.slots, b.ccount, b.cfirst,
+                                                 b.wid_int, b.size2, m, b.rec64, r32, b.n_rec,
+                                                 b.lvl_start, b.by_level);
     }
 }
 

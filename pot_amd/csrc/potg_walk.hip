@@ -118,7 +118,7 @@ __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, 
 }
 
 // ---- float64 walk (field or potential) --------------------------------
-template <int MODE, int KIND>
+template <int MODE, int KIND, bool DIP>
 __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec64* __restrict__ rec,
                                                 WalkIO io)
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
             const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
-            const bool dip = P[gi].dipoles && io.dip64;
+            const bool dip = DIP && P[gi].dipoles && io.dip64;
             uint32_t tf = gi, tc = 1;
             uint64_t tm = grp;
             int sp = 0;
@@ -390,7 +390,7 @@ __device__ inline bool interact32(float tx, float ty, float tz, uint32_t self, f
     return r.cnt != 0;
 }
 
-template <int MODE>
+template <int MODE, bool DIP, bool PAIR>
 __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec32* __restrict__ rec,
                                                 WalkIO io, uint32_t pair_max)
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
             const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
-            const bool dip = P[gi].dipoles && io.dip32;
+            const bool dip = DIP && P[gi].dipoles && io.dip32;
             uint32_t tf = gi, tc = 1;
             uint64_t tm = grp;
             int sp = 0;
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                 const uint32_t m = __popcll(tm);
                 // a pair step pushes up to tc ranges; child steps push at most
                 // one per level below, so MAXD + 1 entries stay in reserve
-                if (m <= pair_max && sp + (int)tc + MAXD + 1 <= STKH) {
+                if (PAIR && m <= pair_max && sp + (int)tc + MAXD + 1 <= STKH) {
                     const uint32_t lsh = tc > 4 ? 3 : (tc > 2 ? 2 : (tc > 1 ? 1 : 0));
                     if ((tm >> lane) & 1ull)
                         s_lane[wv][__popcll(tm & ((1ull << lane) - 1ull))] = (uint8_t)lane;
@@ -601,6 +601,178 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
     }
 }
 
+// ---- grouped float32 walk ----------------------------------------------
+//
+// A wavefront serves a group of K targets (consecutive in key order, so
+// close in space) and evaluates S = 64 / K children of a range at once:
+// lane = slot * K + t pairs target t with child slot `slot`.  A range of
+// up to 8 children takes ceil(cnt / S) passes; each lane fetches its own
+// child record (S distinct records per pass, each shared by K lanes).
+// The per-target MAC is the reference's, so every target gets exactly its
+// own interaction set; a child opened by some of the group's targets is
+// pushed once with the mask of exactly those targets.  Fewer targets per
+// wavefront means the group's interaction lists overlap more, so fewer
+// lanes idle than with 64 targets stepping through one child at a time;
+// the group width K is chosen by potg_tune.
+//
+// Each lane sums its own (target, slot) contributions in float32, flushed
+// every 4 ranges into a compensated sum; at the end the slots of a target
+// are added in float64 across lanes.
+template <int K, int MODE, bool DIP>
+__global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, Globals* g,
+                                               PState st, const NodeRec32* __restrict__ rec,
+                                               WalkIO io)
+{
+    static_assert(K == 8 || K == 16, "group width");
+    constexpr uint32_t S = WAVE / K;
+    constexpr uint32_t KMASK = (1u << K) - 1u;
+    // a visit pops one range and pushes at most 8, so the stack holds at
+    // most 7 per level of the path below the root plus one
+    constexpr int STKG = 7 * MAXD + 8;
+    __shared__ uint32_t s_first[NW][STKG];
+    __shared__ uint32_t s_cm[NW][STKG];  // cnt << 16 | target mask
+    const int wv = threadIdx.x / WAVE;
+    const uint32_t lane = __lane_id();
+    const uint32_t t = lane % K, slot = lane / K;
+    const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
+        tile = rfl(__shfl(tile, 0, WAVE));
+        if ((uint64_t)tile * K >= ntg) break;
+        const uint32_t k = tile * K + t;
+        uint32_t inst, self;
+        double dtx, dty, dtz;
+        const bool valid = load_target<MODE>(P, st, io, k, ntg, inst, self, dtx, dty, dtz);
+        const InstParams& p = P[inst];
+        const float tx = (float)((dtx - p.R) * p.invR);
+        const float ty = (float)((dty - p.R) * p.invR);
+        const float tz = (float)((dtz - p.R) * p.invR);
+        const float theta2 = (float)p.theta2;
+        const float soft2 = (float)(p.soft2 * p.invR * p.invR);
+        float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        uint32_t ncell = 0, nleaf = 0, nopen = 0;
+
+        // the group's targets by instance (slot-0 lanes stand for them)
+        uint32_t pending = (uint32_t)ballot(valid && slot == 0) & KMASK;
+        while (pending) {
+            const uint32_t lead = __ffs(pending) - 1;
+            const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
+            const uint32_t grp = (uint32_t)ballot(valid && slot == 0 && inst == gi) & KMASK;
+            pending &= ~grp;
+            const bool dip = DIP && P[gi].dipoles && io.dip32;
+            uint32_t tf = gi, tc = 1, tm = grp;
+            int sp = 0;
+            uint32_t loads = 0, visits = 0;
+            bool fail = false;
+            for (;;) {
+                if (tc == 0) {
+                    if (sp == 0) break;
+                    --sp;
+                    tf = rfl(s_first[wv][sp]);
+                    const uint32_t cm = rfl(s_cm[wv][sp]);
+                    tc = cm >> 16;
+                    tm = cm & KMASK;
+                }
+                if (++loads > io.n_rec || tc > 8) {
+                    raise_err(g, tc > 8 ? KERR_CAPACITY : KERR_DEPTH, tf);
+                    break;
+                }
+                tot_pops++;
+                const bool mine = (tm >> t) & 1u;
+                for (uint32_t c0 = 0; c0 < tc; c0 += S) {
+                    const uint32_t c = c0 + slot;
+                    bool op = false;
+                    uint32_t cf = 0, cn = 0;
+                    if (mine && c < tc) {
+                        const NodeRec32 r = rec[tf + c];
+                        float4 pd = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (DIP && dip) pd = io.dip32[tf + c];
+                        op = interact32(tx, ty, tz, self, theta2, soft2, r, DIP && dip, pd, px, py,
+                                        pz, ncell, nleaf);
+                        cf = r.first;
+                        cn = r.cnt;
+                    }
+                    if (op) nopen++;
+                    const uint64_t b = ballot(op);
+                    // push this pass's opened children, ascending: the
+                    // youngest ends on top and is visited first
+                    for (uint32_t s2 = 0; s2 < S && c0 + s2 < tc; s2++) {
+                        const uint32_t om = (uint32_t)(b >> (s2 * K)) & KMASK;
+                        if (!om) continue;
+                        const uint32_t src = s2 * K + (__ffs(om) - 1);
+                        const uint32_t ccf = (uint32_t)__builtin_amdgcn_readlane((int)cf, src);
+                        const uint32_t ccn = (uint32_t)__builtin_amdgcn_readlane((int)cn, src);
+                        if (ccf <= tf + c0 + s2 || ccn > 8 || ccf + ccn > io.n_rec || sp >= STKG) {
+                            raise_err(g, sp >= STKG ? KERR_DEPTH : KERR_CAPACITY, tf + c0 + s2);
+                            fail = true;
+                            break;
+                        }
+                        if (lane == 0) {
+                            s_first[wv][sp] = ccf;
+                            s_cm[wv][sp] = (ccn << 16) | om;
+                        }
+                        sp++;
+                    }
+                    if (fail) break;
+                }
+                if (fail) break;
+                tc = 0;
+                if ((++visits & 3u) == 0) {
+                    kahan_add(fx, kx, px);
+                    kahan_add(fy, ky, py);
+                    kahan_add(fz, kz, pz);
+                    px = py = pz = 0.f;
+                }
+            }
+        }
+        kahan_add(fx, kx, px);
+        kahan_add(fy, ky, py);
+        kahan_add(fz, kz, pz);
+        // add the slots of each target (lanes t, t + K, ...) in float64
+        double ex = (double)fx - (double)kx, ey = (double)fy - (double)ky,
+               ez = (double)fz - (double)kz;
+        for (uint32_t d = K; d < WAVE; d <<= 1) {
+            ex += __shfl_xor(ex, (int)d, WAVE);
+            ey += __shfl_xor(ey, (int)d, WAVE);
+            ez += __shfl_xor(ez, (int)d, WAVE);
+            ncell += __shfl_xor(ncell, (int)d, WAVE);
+            nleaf += __shfl_xor(nleaf, (int)d, WAVE);
+            nopen += __shfl_xor(nopen, (int)d, WAVE);
+        }
+        if (valid && slot == 0) {
+            const double Kf = QE / (K4PE0 * p.R * p.R);
+            io.out[3 * k] += Kf * ex;
+            io.out[3 * k + 1] += Kf * ey;
+            io.out[3 * k + 2] += Kf * ez;
+            if (io.counts) {
+                io.counts[3 * k] = ncell;
+                io.counts[3 * k + 1] = nleaf;
+                io.counts[3 * k + 2] = nopen;
+            }
+        }
+        if (slot == 0) {
+            tot_cell += ncell;
+            tot_leaf += nleaf;
+            tot_open += nopen;
+        }
+    }
+    for (int d = 1; d < WAVE; d <<= 1) {
+        tot_cell += __shfl_xor(tot_cell, d, WAVE);
+        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+        tot_open += __shfl_xor(tot_open, d, WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd(&g->n_cell, tot_cell);
+        atomicAdd(&g->n_leaf, tot_leaf);
+        atomicAdd(&g->n_open, tot_open);
+        atomicAdd(&g->n_pops, tot_pops);
+    }
+}
+
 // initial accumulator of every target: the spheroid grain's analytic field
 // (SPH:1815-1818 seeds E with it); for phi_at, the grain potential at the
 // absolute coordinates as phi_at uses them (SPH:1924-1953); for the
@@ -675,13 +847,47 @@ void launch_walk_init(Bufs& b, hipStream_t s)
     k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
 }
 
+// float32 field walk of the chosen shape: groups of 8 or 16 targets
+// (k_walkg), or 64 targets stepping through the children with or without
+// the pair mode (k_walk32); instantiated with dipole support only for
+// contexts holding DIPOLES instances
+template <int MODE>
+static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
+{
+    const bool dip = b.dip32 != nullptr;
+#define POTG_W32(KERNEL, ...) KERNEL<<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io __VA_ARGS__)
+    if (b.group == 8) {
+        if (dip) POTG_W32((k_walkg<8, MODE, true>));
+        else POTG_W32((k_walkg<8, MODE, false>));
+    } else if (b.group == 16) {
+        if (dip) POTG_W32((k_walkg<16, MODE, true>));
+        else POTG_W32((k_walkg<16, MODE, false>));
+    } else if (b.pair_max) {
+        if (dip) POTG_W32((k_walk32<MODE, true, true>), , b.pair_max);
+        else POTG_W32((k_walk32<MODE, false, true>), , b.pair_max);
+    } else {
+        if (dip) POTG_W32((k_walk32<MODE, true, false>), , 0u);
+        else POTG_W32((k_walk32<MODE, false, false>), , 0u);
+    }
+#undef POTG_W32
+}
+
+template <int MODE, int KIND>
+static void launch_walk64(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
+{
+    if (b.dip64)
+        k_walk64<MODE, KIND, true><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    else
+        k_walk64<MODE, KIND, false><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+}
+
 void launch_walk(Bufs& b, hipStream_t s)
 {
     WalkIO io = step_io(b);
     if (b.precision)
-        k_walk32<MODE_STEP><<<b.walk_blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
+        launch_walk32<MODE_STEP>(b, (int)b.walk_blocks, io, s);
     else
-        k_walk64<MODE_STEP, KIND_FIELD><<<b.walk_blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+        launch_walk64<MODE_STEP, KIND_FIELD>(b, (int)b.walk_blocks, io, s);
 }
 
 void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
@@ -702,13 +908,13 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     const int blocks = (int)nb < (int)b.walk_blocks ? (int)nb : (int)b.walk_blocks;
     if (kind == 1) {
         k_grain_init<MODE_POINTS, KIND_PHI><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
-        k_walk64<MODE_POINTS, KIND_PHI><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+        launch_walk64<MODE_POINTS, KIND_PHI>(b, blocks, io, s);
     } else {
         k_grain_init<MODE_POINTS, KIND_FIELD><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
         if (b.precision)
-            k_walk32<MODE_POINTS><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io, b.pair_max);
+            launch_walk32<MODE_POINTS>(b, blocks, io, s);
         else
-            k_walk64<MODE_POINTS, KIND_FIELD><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+            launch_walk64<MODE_POINTS, KIND_FIELD>(b, blocks, io, s);
     }
 }
 
@@ -760,7 +966,7 @@ void launch_epe(Bufs& b, hipStream_t s)
     const uint32_t nb = (b.n_total + BT - 1) / BT;
     const int blocks = (int)nb < (int)b.walk_blocks ? (int)nb : (int)b.walk_blocks;
     k_grain_init<MODE_EPE, KIND_EPE><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
-    k_walk64<MODE_EPE, KIND_EPE><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    launch_walk64<MODE_EPE, KIND_EPE>(b, blocks, io, s);
     k_epe_partial<<<dim3(EPE_BLOCKS, b.n_inst), BT, 0, s>>>(b.P, b.st, b.phi, b.epe_part);
 }
 
