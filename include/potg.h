@@ -50,6 +50,12 @@ extern "C" {
 /* arithmetic of the field walk */
 #define POTG_FP64 0  /* float64 everywhere */
 #define POTG_MIXED 1 /* float32 walk with compensated sums; float64 push */
+/* As POTG_MIXED, but the 64 targets of a wavefront share one interaction
+   list built with the opening criterion applied to their bounding box:
+   every cell the reference accepts for a target is taken whole or in
+   parts, so forces are at least as accurate; no counterpart in the
+   reference. */
+#define POTG_MIXED_GROUP 2
 
 /* The reference's 56-byte AoS particle record (pot.c:295-303); also the
    record format of state files (pot.c:3888). */

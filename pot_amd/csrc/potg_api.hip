@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/potg.h"
@@ -267,7 +268,7 @@ const char* potg_last_error(void) { return g_err; }
 int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, potg_ctx** out)
 {
     if (!configs || !out || n_inst == 0 || n_inst > 65535 ||
-        (precision != POTG_FP64 && precision != POTG_MIXED))
+        (precision != POTG_FP64 && precision != POTG_MIXED && precision != POTG_MIXED_GROUP))
         return set_err("potg_create: bad arguments"), POTG_ERR_ARG;
     *out = nullptr;
     uint64_t n_total = 0, n_tree = 0;
@@ -356,13 +357,13 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
 
     rc |= alloc_n(c, &b.rec64, b.n_rec);
-    if (precision == POTG_MIXED) rc |= alloc_n(c, &b.rec32, b.n_rec);
+    if (precision != POTG_FP64) rc |= alloc_n(c, &b.rec32, b.n_rec);
     bool any_dip = false;
     for (uint32_t i = 0; i < n_inst; i++) any_dip |= configs[i].dipoles != 0;
     if (any_dip) {
         rc |= alloc_n(c, &b.mpx, ci) | alloc_n(c, &b.mpy, ci) | alloc_n(c, &b.mpz, ci);
         rc |= alloc_n(c, &b.dip64, 3 * (size_t)b.n_rec);
-        if (precision == POTG_MIXED) rc |= alloc_n(c, &b.dip32, b.n_rec);
+        if (precision != POTG_FP64) rc |= alloc_n(c, &b.dip32, b.n_rec);
     }
     rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
     rc |= alloc_n(c, &b.queue, nt);
@@ -694,12 +695,54 @@ static int run_points(potg_ctx* c, uint32_t inst, const double* xyz, const uint3
     return kernel_error(c);
 }
 
+static uint64_t spread3(uint64_t v)  // 21 bits to every third of 63
+{
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
 int potg_field_at(potg_ctx* c, uint32_t inst, const double* xyz, const uint32_t* exclude, size_t n,
                   double* E, uint32_t* counts)
 {
     if (!c || inst >= c->b.n_inst || (n && (!xyz || !E)))
         return set_err("potg_field_at: bad arguments"), POTG_ERR_ARG;
-    return run_points(c, inst, xyz, exclude, n, 0, E, counts);
+    if (c->b.precision != POTG_MIXED_GROUP || n < 2)
+        return run_points(c, inst, xyz, exclude, n, 0, E, counts);
+    // the group criterion wants the 64 points of a wavefront close together:
+    // walk the points in Morton order of their positions, then put the
+    // results back in the caller's order
+    const double scale = (double)(1u << 21) / c->hp[inst].root_x2;
+    std::vector<std::pair<uint64_t, size_t>> ord(n);
+    for (size_t k = 0; k < n; k++) {
+        uint64_t u[3];
+        for (int d = 0; d < 3; d++) {
+            double f = xyz[3 * k + d] * scale;
+            f = f < 0.0 ? 0.0 : (f > 2097151.0 ? 2097151.0 : f);
+            u[d] = (uint64_t)f;
+        }
+        ord[k] = {spread3(u[0]) << 2 | spread3(u[1]) << 1 | spread3(u[2]), k};
+    }
+    std::sort(ord.begin(), ord.end());
+    std::vector<double> pxyz(3 * n), pE(3 * n);
+    std::vector<uint32_t> pex(exclude ? n : 0), pcnt(counts ? 3 * n : 0);
+    for (size_t k = 0; k < n; k++) {
+        for (int d = 0; d < 3; d++) pxyz[3 * k + d] = xyz[3 * ord[k].second + d];
+        if (exclude) pex[k] = exclude[ord[k].second];
+    }
+    const int rc = run_points(c, inst, pxyz.data(), exclude ? pex.data() : nullptr, n, 0, pE.data(),
+                              counts ? pcnt.data() : nullptr);
+    if (rc) return rc;
+    for (size_t k = 0; k < n; k++)
+        for (int d = 0; d < 3; d++) {
+            E[3 * ord[k].second + d] = pE[3 * k + d];
+            if (counts) counts[3 * ord[k].second + d] = pcnt[3 * k + d];
+        }
+    return POTG_OK;
 }
 
 int potg_phi_at(potg_ctx* c, uint32_t inst, const double* xyz, size_t n, double* phi)
@@ -814,7 +857,8 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     for (uint32_t wb : bpc) {
         for (const auto& cd : cand) {
-            if (!b.precision && (cd[0] != 64 || cd[1] != 0)) continue;
+            // the float64 and group walks have one shape: only the grid varies
+            if (b.precision != POTG_MIXED && (cd[0] != 64 || cd[1] != 0)) continue;
             b.group = cd[0];
             b.pair_max = cd[1];
             b.walk_blocks = wb * cus;

@@ -2,6 +2,7 @@
 // of the step pipeline.  Not part of the C ABI.
 #pragma once
 
+#include "../../include/potg.h"
 #include "potg_dev.h"
 
 namespace potg {
@@ -27,7 +28,7 @@ struct Bufs {
     uint32_t n_total;  // particle slots over all instances
     uint32_t n_tree;   // leading sorted slots that belong to the trees
     uint32_t n_inst;
-    int precision;     // POTG_FP64 / POTG_MIXED
+    int precision;     // POTG_FP64 / POTG_MIXED / POTG_MIXED_GROUP
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
     uint32_t pair_max; // walk: ranges with at most this many targets go pair-parallel

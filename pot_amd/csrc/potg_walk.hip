@@ -773,6 +773,260 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
     }
 }
 
+// ---- float32 walk with the group opening criterion --------------------
+//
+// (POTG_MIXED_GROUP)  The 64 targets of a wavefront share one interaction
+// list.  A cell is accepted for all of them when size^2 < theta^2 dmin^2,
+// dmin being the distance from the cell's centre of |q| to the bounding box
+// of the targets; since dmin is at most each target's own distance, every
+// cell accepted here the reference would accept for every target too, so
+// each target's interactions refine the reference's (cells it would have
+// taken whole are taken in parts): the force is at least as accurate, and
+// the lanes are never idle.
+//
+// A wavefront's 64 targets are consecutive in key order, which can cross
+// from one large cell into a distant one; so the targets are split into
+// groups by the octree cell holding them at depth D, chosen so that such a
+// cell holds a few dozen targets, and each group gets its own list.
+//
+// Phase A builds the list lane-parallel: up to 8 child ranges are taken
+// off a stack at once, one child per lane, each lane applying the group
+// criterion to its child; accepted cells and leaves are appended to an LDS
+// list, opened cells' child ranges pushed.  Phase B, whenever the list
+// fills or the walk ends, runs the list: every lane adds the interaction of
+// each entry with its own target, the entry's record coming in through
+// scalar loads.  The target's own leaf is skipped (pot.c:1787).
+template <int MODE, bool DIP>
+__global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__ P, Globals* g,
+                                                   PState st, const NodeRec32* __restrict__ rec,
+                                                   WalkIO io)
+{
+    constexpr int NS = 1024;  // child ranges waiting: first | (cnt - 1) << 29
+    constexpr int NL = 512;   // interaction list
+    // while the stack is below this, up to 8 ranges are expanded at a time;
+    // above it one (pure depth first), which adds at most 7 per level
+    constexpr int SP_WIDE = NS - 7 * MAXD - 72;
+    __shared__ uint32_t s_stk[NW][NS];
+    __shared__ uint32_t s_list[NW][NL];
+    const int wv = threadIdx.x / WAVE;
+    const uint32_t lane = __lane_id();
+    const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
+        tile = rfl(__shfl(tile, 0, WAVE));
+        if ((uint64_t)tile * WAVE >= ntg) break;
+        const uint32_t k = tile * WAVE + lane;
+        uint32_t inst, self;
+        double dtx, dty, dtz;
+        const bool valid = load_target<MODE>(P, st, io, k, ntg, inst, self, dtx, dty, dtz);
+        const InstParams& p = P[inst];
+        const float tx = (float)((dtx - p.R) * p.invR);
+        const float ty = (float)((dty - p.R) * p.invR);
+        const float tz = (float)((dtz - p.R) * p.invR);
+        const float soft2 = (float)(p.soft2 * p.invR * p.invR);
+        float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        uint32_t ncell = 0, nleaf = 0, nopen = 0;
+        // the target's octree cell at depth D: D = log8(tree particles / 32)
+        uint32_t cell = 0;
+        {
+            const uint32_t nt = p.tree_n > 64u ? p.tree_n : 64u;
+            int D = (31 - __clz(nt / 32u)) / 3;
+            D = D < 1 ? 1 : (D > 9 ? 9 : D);
+            const float sc = (float)(1u << D) * 0.5f;
+            const uint32_t mx = (1u << D) - 1u;
+            const uint32_t cx = (uint32_t)fminf(fmaxf((tx + 1.f) * sc, 0.f), (float)mx);
+            const uint32_t cy = (uint32_t)fminf(fmaxf((ty + 1.f) * sc, 0.f), (float)mx);
+            const uint32_t cz = (uint32_t)fminf(fmaxf((tz + 1.f) * sc, 0.f), (float)mx);
+            cell = (cx << 20) | (cy << 10) | cz;
+        }
+
+        uint64_t pending = ballot(valid);
+        while (pending) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
+            const uint32_t gc = rfl(__shfl(cell, lead, WAVE));
+            const bool mine = valid && inst == gi && cell == gc;
+            pending &= ~rfl64(ballot(mine));
+            const bool dip = DIP && P[gi].dipoles && io.dip32;
+            const float theta2 = (float)P[gi].theta2;
+            // bounding box of this instance's targets in the wavefront
+            float lx = mine ? tx : 3.f, ly = mine ? ty : 3.f, lz = mine ? tz : 3.f;
+            float hx = mine ? tx : -3.f, hy = mine ? ty : -3.f, hz = mine ? tz : -3.f;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                lx = fminf(lx, __shfl_xor(lx, d, WAVE));
+                ly = fminf(ly, __shfl_xor(ly, d, WAVE));
+                lz = fminf(lz, __shfl_xor(lz, d, WAVE));
+                hx = fmaxf(hx, __shfl_xor(hx, d, WAVE));
+                hy = fmaxf(hy, __shfl_xor(hy, d, WAVE));
+                hz = fmaxf(hz, __shfl_xor(hz, d, WAVE));
+            }
+            // the root's children: the root record is a leaf only for a
+            // one-particle tree; an empty instance has no records to walk
+            int sp = 0, nl = 0;
+            {
+                const NodeRec32 r0 = rec[gi];
+                if (r0.cnt == LEAF) {
+                    if (lane == 0) s_list[wv][0] = gi;
+                    nl = 1;
+                } else if (r0.cnt != 0) {
+                    if (r0.cnt - 1u > 7u || r0.first + r0.cnt > io.n_rec) {
+                        raise_err(g, KERR_CAPACITY, gi);
+                        continue;
+                    }
+                    if (lane == 0) s_stk[wv][0] = r0.first | ((r0.cnt - 1u) << 29);
+                    sp = 1;
+                    nopen++;
+                }
+            }
+            float px = 0.f, py = 0.f, pz = 0.f;
+            uint32_t loads = 0;
+            bool fail = false;
+            while (sp > 0 || nl > 0) {
+                // ---- phase A: expand ranges until the list is nearly full
+                while (sp > 0 && nl <= NL - WAVE) {
+                    if (++loads > io.n_rec) {
+                        raise_err(g, KERR_DEPTH, gi);
+                        fail = true;
+                        break;
+                    }
+                    const int take = sp > SP_WIDE ? 1 : (sp < 8 ? sp : 8);
+                    tot_pops += (unsigned long long)take;
+                    const uint32_t ri = lane >> 3, ci = lane & 7u;  // range, child
+                    uint32_t w = NONE;
+                    if ((int)ri < take) {
+                        const uint32_t fc = s_stk[wv][sp - 1 - (int)ri];
+                        if (ci <= (fc >> 29)) w = (fc & 0x1fffffffu) + ci;
+                    }
+                    sp -= take;
+                    bool to_list = false, to_stack = false;
+                    uint32_t cf = 0, cn = 0;
+                    if (w != NONE) {
+                        const NodeRec32 r = rec[w];
+                        if (r.cnt == LEAF) {
+                            to_list = true;
+                        } else {
+                            const float ex = fmaxf(0.f, fmaxf(lx - r.cx, r.cx - hx));
+                            const float ey = fmaxf(0.f, fmaxf(ly - r.cy, r.cy - hy));
+                            const float ez = fmaxf(0.f, fmaxf(lz - r.cz, r.cz - hz));
+                            const float dmin2 = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez));
+                            if (r.size2 < theta2 * dmin2)
+                                to_list = dip ? r.part != 0u : r.q != 0.f;
+                            else if (r.cnt != 0) {
+                                to_stack = true;
+                                cf = r.first;
+                                cn = r.cnt;
+                            }
+                        }
+                    }
+                    if (to_stack && (cn - 1u > 7u || cf <= w || cf + cn > io.n_rec)) {
+                        raise_err(g, KERR_CAPACITY, w);
+                        to_stack = false;
+                    }
+                    const uint64_t bl = ballot(to_list), bs = ballot(to_stack);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(bl >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bl, 0u));
+                    if (to_list) s_list[wv][nl + (int)below] = w;
+                    nl += __popcll(bl);
+                    const int ns = __popcll(bs);
+                    if (sp + ns > NS) {
+                        raise_err(g, KERR_DEPTH, gi);
+                        fail = true;
+                        break;
+                    }
+                    const uint32_t sbelow = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(bs >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bs, 0u));
+                    if (to_stack) s_stk[wv][sp + (int)sbelow] = cf | ((cn - 1u) << 29);
+                    sp += ns;
+                    nopen += (uint32_t)ns;
+                }
+                if (fail) break;
+                __builtin_amdgcn_wave_barrier();
+                // ---- phase B: every target against every listed entry,
+                // the next entry's record requested before this one is used
+                uint32_t wn = nl > 0 ? rfl(s_list[wv][0]) : 0u;
+                NodeRec32 rn = rec[wn];  // uniform: scalar loads
+                for (int e = 0; e < nl; e++) {
+                    const uint32_t w = wn;
+                    const NodeRec32 r = rn;
+                    if (e + 1 < nl) {
+                        wn = rfl(s_list[wv][e + 1]);
+                        rn = rec[wn];
+                    }
+                    const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+                    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    if (r.cnt == LEAF) {
+                        const bool use = mine && r.part != self && d2 != 0.f;
+                        const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
+                        const float sc = use ? r.q * (inv * inv * inv) : 0.f;
+                        px = __builtin_fmaf(sc, dx, px);
+                        py = __builtin_fmaf(sc, dy, py);
+                        pz = __builtin_fmaf(sc, dz, pz);
+                        nleaf += use ? 1u : 0u;
+                    } else {
+                        const float inv = __builtin_amdgcn_rsqf(d2);
+                        const float inv2 = inv * inv;
+                        const float inv3 = mine ? inv2 * inv : 0.f;
+                        if (DIP && dip) {
+                            const float4 pd = io.dip32[w];
+                            const float rpd2 =
+                                __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
+                            const float qq = r.q + 3.f * rpd2;
+                            px = __builtin_fmaf(inv3, __builtin_fmaf(qq, dx, -pd.x), px);
+                            py = __builtin_fmaf(inv3, __builtin_fmaf(qq, dy, -pd.y), py);
+                            pz = __builtin_fmaf(inv3, __builtin_fmaf(qq, dz, -pd.z), pz);
+                        } else {
+                            const float sc = r.q * inv3;
+                            px = __builtin_fmaf(sc, dx, px);
+                            py = __builtin_fmaf(sc, dy, py);
+                            pz = __builtin_fmaf(sc, dz, pz);
+                        }
+                        ncell += mine ? 1u : 0u;
+                    }
+                    if ((e & 15) == 15) {
+                        kahan_add(fx, kx, px);
+                        kahan_add(fy, ky, py);
+                        kahan_add(fz, kz, pz);
+                        px = py = pz = 0.f;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                nl = 0;
+            }
+            kahan_add(fx, kx, px);
+            kahan_add(fy, ky, py);
+            kahan_add(fz, kz, pz);
+        }
+        if (valid) {
+            const double Kf = QE / (K4PE0 * p.R * p.R);
+            io.out[3 * k] += Kf * ((double)fx - (double)kx);
+            io.out[3 * k + 1] += Kf * ((double)fy - (double)ky);
+            io.out[3 * k + 2] += Kf * ((double)fz - (double)kz);
+            if (io.counts) {
+                io.counts[3 * k] = ncell;
+                io.counts[3 * k + 1] = nleaf;
+                io.counts[3 * k + 2] = nopen;
+            }
+        }
+        tot_cell += valid ? ncell : 0u;
+        tot_leaf += valid ? nleaf : 0u;
+        tot_open += lane == 0 ? nopen : 0u;
+    }
+    for (int d = 1; d < WAVE; d <<= 1) {
+        tot_cell += __shfl_xor(tot_cell, d, WAVE);
+        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+        tot_open += __shfl_xor(tot_open, d, WAVE);
+    }
+    if (lane == 0) {
+        atomicAdd(&g->n_cell, tot_cell);
+        atomicAdd(&g->n_leaf, tot_leaf);
+        atomicAdd(&g->n_open, tot_open);
+        atomicAdd(&g->n_pops, tot_pops);
+    }
+}
+
 // initial accumulator of every target: the spheroid grain's analytic field
 // (SPH:1815-1818 seeds E with it); for phi_at, the grain potential at the
 // absolute coordinates as phi_at uses them (SPH:1924-1953); for the
@@ -856,7 +1110,10 @@ static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
 {
     const bool dip = b.dip32 != nullptr;
 #define POTG_W32(KERNEL, ...) KERNEL<<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io __VA_ARGS__)
-    if (b.group == 8) {
+    if (b.precision == POTG_MIXED_GROUP) {
+        if (dip) POTG_W32((k_walk32grp<MODE, true>));
+        else POTG_W32((k_walk32grp<MODE, false>));
+    } else if (b.group == 8) {
         if (dip) POTG_W32((k_walkg<8, MODE, true>));
         else POTG_W32((k_walkg<8, MODE, false>));
     } else if (b.group == 16) {

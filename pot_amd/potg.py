@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libpotg.so")
 
 POTG_SPHERE, POTG_PROLATE, POTG_OBLATE = 0, 1, 2
-POTG_FP64, POTG_MIXED = 0, 1
+POTG_FP64, POTG_MIXED, POTG_MIXED_GROUP = 0, 1, 2
 DUST, ION, ELECTRON = 0, 1, 2
 
 PARTICLE_DT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),

@@ -228,6 +228,58 @@ def test_mixed_walk_shapes_agree(case, group, pair_max):
     assert np.median(rel) < 1e-5
 
 
+def _rel(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+@pytest.mark.parametrize("case", [BIG[0], BIG[1], BIG[3]], ids=["sphere", "prolate", "dipoles"])
+def test_group_walk_refines_the_reference(case):
+    """POTG_MIXED_GROUP: each target's interactions refine the reference's
+    (cells taken whole by the reference may be taken in parts), so against
+    the softened float64 direct sum its errors are smaller than the
+    reference walk's."""
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    ref = gpu_ctx(case, parts)
+    ref.build_tree()
+    grp = gpu_ctx(case, parts, precision=G.POTG_MIXED_GROUP)
+    grp.build_tree()
+    idx = np.arange(1, len(parts), 5, dtype=np.uint32)
+    xyz = np.stack([parts["x"][idx], parts["y"][idx], parts["z"][idx]], 1)
+    Ed = ref.direct_sum(0, idx)
+    e_ref = _rel(ref.field_at(0, xyz, exclude=idx), Ed)
+    e_grp = _rel(grp.field_at(0, xyz, exclude=idx), Ed)
+    assert np.median(e_grp) < np.median(e_ref), (np.median(e_grp), np.median(e_ref))
+    assert np.percentile(e_grp, 90) < np.percentile(e_ref, 90)
+
+
+def test_group_walk_at_theta_zero_is_the_direct_sum():
+    """At theta = 0 no cell passes the criterion: the group walk is the
+    softened direct sum over all other particles, in float32."""
+    case = dict(BIG[0], theta=0.0)
+    parts = oracle_sim(case).particles()
+    grp = gpu_ctx(case, parts, precision=G.POTG_MIXED_GROUP)
+    grp.build_tree()
+    idx = np.arange(1, len(parts), 11, dtype=np.uint32)
+    xyz = np.stack([parts["x"][idx], parts["y"][idx], parts["z"][idx]], 1)
+    e = _rel(grp.field_at(0, xyz, exclude=idx), grp.direct_sum(0, idx))
+    assert np.median(e) < 1e-5 and e.max() < 1e-3, (np.median(e), e.max())
+
+
+def test_group_walk_steps_and_tunes():
+    case = BIG[1]
+    parts = oracle_sim(case).particles()
+    ctx = gpu_ctx(case, parts, precision=G.POTG_MIXED_GROUP, seed=9)
+    for _ in range(3):
+        ctx.step()
+    t = ctx.tune()
+    assert t["walk_ms"] > 0
+    ctx.step()
+    p, _ = ctx.download(0)
+    assert np.array_equal(p["species"], parts["species"])
+    assert np.all(np.isfinite(p["x"])) and ctx.stats()["steps"] == 4
+
+
 def test_graph_replay_equals_eager_steps():
     case = big_case(20001, seed=21)
     parts = oracle_sim(case).particles()
