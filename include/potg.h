@@ -126,15 +126,12 @@ typedef struct potg_stats_t {
 
 /* Launch parameters of the field walk (potg_tune picks them by timing). */
 typedef struct potg_tuning {
-    uint32_t pair_max;    /* child ranges whose mask holds at most this many
-                             targets are evaluated as target x child pairs
-                             (0..64; float32 walk only) */
     uint32_t walk_blocks; /* blocks (4 wavefronts each) of the persistent walk */
     uint32_t group;       /* float32 walk: targets per wavefront, 8, 16 or 64;
                              with 8 or 16 the wavefront evaluates 8 or 4
-                             children per target at once (pair_max unused) */
-    uint32_t pad;
-    double walk_ms;       /* potg_tune: time of the chosen walk on the next step */
+                             children per target at once */
+    uint32_t pad[2];
+    double walk_ms;       /* potg_tune: the chosen walk's time (ms) */
 } potg_tuning;
 
 typedef struct potg_ctx potg_ctx;

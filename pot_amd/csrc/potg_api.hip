@@ -307,9 +307,8 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
     b.key_bits = 64;
-    // defaults until potg_tune or potg_set_tuning: ranges with at most 16
-    // targets go pair-parallel; 8 blocks of 4 waves per CU
-    b.pair_max = 16;
+    // defaults until potg_tune or potg_set_tuning: 64 targets per
+    // wavefront, 8 blocks of 4 waves per CU
     b.walk_blocks = 8 * device_cus();
     b.group = 64;
 
@@ -355,6 +354,13 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.mcx, ci) | alloc_n(c, &b.mcy, ci) | alloc_n(c, &b.mcz, ci);
     rc |= alloc_n(c, &b.mq, ci) | alloc_n(c, &b.maq, ci);
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
+    // the group walk packs a child range as first | (cnt - 1) << 29
+    if (!rc && precision == POTG_MIXED_GROUP && b.n_rec >= (1u << 29)) {
+        potg_destroy(c);
+        return set_err("potg_create: POTG_MIXED_GROUP holds at most %u tree records "
+                       "(about 178M particles)", (1u << 29) - 1),
+               POTG_ERR_ARG;
+    }
 
     rc |= alloc_n(c, &b.rec64, b.n_rec);
     if (precision != POTG_FP64) rc |= alloc_n(c, &b.rec32, b.n_rec);
@@ -808,21 +814,18 @@ int potg_set_tuning(potg_ctx* c, const potg_tuning* set, potg_tuning* get)
 {
     if (!c) return set_err("potg_set_tuning: null context"), POTG_ERR_ARG;
     if (set) {
-        if (set->pair_max > 64 || set->walk_blocks == 0 || set->walk_blocks > 65536 ||
+        if (set->walk_blocks == 0 || set->walk_blocks > 65536 ||
             (set->group != 8 && set->group != 16 && set->group != 64))
-            return set_err("potg_set_tuning: pair_max must be 0..64, walk_blocks 1..65536, "
-                           "group 8, 16 or 64"),
+            return set_err("potg_set_tuning: walk_blocks must be 1..65536, group 8, 16 or 64"),
                    POTG_ERR_ARG;
-        c->b.pair_max = set->pair_max;
         c->b.walk_blocks = set->walk_blocks;
         c->b.group = set->group;
         destroy_graphs(c);  // captured steps hold the old launch parameters
     }
     if (get) {
-        get->pair_max = c->b.pair_max;
         get->walk_blocks = c->b.walk_blocks;
         get->group = c->b.group;
-        get->pad = 0;
+        get->pad[0] = get->pad[1] = 0;
         get->walk_ms = c->tuned_ms;
     }
     return POTG_OK;
@@ -847,20 +850,19 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     rc = kernel_error(c);
     if (rc) return rc;
     const uint32_t cus = (uint32_t)device_cus();
-    // candidates: (group, pair threshold); the float64 walk has neither
-    const uint32_t cand[][2] = {{8, 0},  {16, 0}, {64, 0},  {64, 4},  {64, 8},
-                                {64, 16}, {64, 32}, {64, 64}};
+    // candidates: targets per wavefront (float32 walk only) x blocks per CU
+    const uint32_t groups[] = {8, 16, 64};
     const uint32_t bpc[] = {4, 8, 16};
-    const uint32_t keep_pm = b.pair_max, keep_wb = b.walk_blocks, keep_gr = b.group;
+    const uint32_t keep_wb = b.walk_blocks, keep_gr = b.group;
     double best = 1e30;
-    uint32_t best_pm = keep_pm, best_wb = keep_wb, best_gr = keep_gr;
+    uint32_t best_wb = keep_wb, best_gr = keep_gr;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     for (uint32_t wb : bpc) {
-        for (const auto& cd : cand) {
-            // the float64 and group walks have one shape: only the grid varies
-            if (b.precision != POTG_MIXED && (cd[0] != 64 || cd[1] != 0)) continue;
-            b.group = cd[0];
-            b.pair_max = cd[1];
+        for (uint32_t gr : groups) {
+            // the float64 and group-criterion walks have one shape: only the
+            // grid varies
+            if (b.precision != POTG_MIXED && gr != 64) continue;
+            b.group = gr;
             b.walk_blocks = wb * cus;
             float ms_best = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
@@ -876,7 +878,6 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
             }
             if (ms_best < best) {
                 best = ms_best;
-                best_pm = b.pair_max;
                 best_wb = b.walk_blocks;
                 best_gr = b.group;
             }
@@ -894,7 +895,6 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     g1.n_tgt_total = g0.n_tgt_total;
     g1.n_steps = g0.n_steps;
     HIPCHK(hipMemcpy(b.g, &g1, sizeof(g1), hipMemcpyHostToDevice));
-    b.pair_max = rc ? keep_pm : best_pm;
     b.walk_blocks = rc ? keep_wb : best_wb;
     b.group = rc ? keep_gr : best_gr;
     c->tuned_ms = rc ? 0.0 : best;

@@ -31,7 +31,6 @@ struct Bufs {
     int precision;     // POTG_FP64 / POTG_MIXED / POTG_MIXED_GROUP
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
-    uint32_t pair_max; // walk: ranges with at most this many targets go pair-parallel
     uint32_t walk_blocks; // blocks of the persistent walk grid
     uint32_t group;    // float32 walk: targets per wavefront (8, 16 or 64)
     InstParams* P;

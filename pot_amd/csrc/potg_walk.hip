@@ -325,26 +325,27 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
     }
 }
 
-// ---- float32 compensated walk (field): child steps or target x child pairs
+// ---- float32 compensated walk (field) ----------------------------------
 //
 // Same traversal and per-particle MAC as k_walk64, in float32 with
 // coordinates relative to the domain centre in units of R.  A fetched child
-// range (<= 8 children, staged in LDS) is evaluated against the targets in
-// its mask in one of two ways:
-//   * child steps (dense masks): the wave visits the children one at a
-//     time, youngest first, each lane testing its own target;
-//   * pairs (sparse masks, at most `pair_max` targets): the m x C
-//     (target, child) pairs, C = cnt rounded up to a power of two, are laid
-//     out densely across the lanes, so a range costs ceil(m C / 64) wave
-//     operations instead of cnt.  Contributions are added per target in
-//     LDS, and every child opened by some targets is pushed with the mask
-//     of exactly those targets, youngest child on top.
-// Either way each target gets exactly the reference's interaction set;
-// only the summation order differs from the depth-first order.  The
-// interactions of one range visit are summed in plain float32 and that sum
-// enters the per-particle Kahan sum.
-constexpr int STKH = 192;
-
+// range (<= 8 children) is staged in LDS and visited in two passes: its
+// leaves first, which every target in the mask takes (but for its own
+// leaf) and which are never opened, then its cells, youngest first, each
+// lane applying the MAC for its own target, until some lanes open a cell:
+// the wavefront descends into it with those lanes, and the cells not yet
+// visited wait on the stack (the leaves are done).  Each target gets
+// exactly the reference's interaction set; only the summation order
+// differs from the depth-first order.  The interactions of one range visit
+// are summed in plain float32 and that sum enters the per-particle Kahan
+// sum.
+//
+// Lane sets are 64-bit scalar masks and every per-child decision that is
+// the same for all lanes (leaf or cell, which child next) is scalar, so a
+// child step costs few scalar instructions besides its vector arithmetic;
+// lanes not taking a child get zero coefficients and every lane adds
+// a * d + b * p to its sums in place.  The wavefront's interaction totals
+// are scalar popcounts; per-lane counts are kept only for point queries.
 // one (target, node) interaction; returns true when the target opens the node.
 // With dip (DIPOLES), cells holding any charge are accepted and pd is the
 // cell's dipole in units of R (pot.c:1732-1745).
@@ -390,25 +391,109 @@ __device__ inline bool interact32(float tx, float ty, float tz, uint32_t self, f
     return r.cnt != 0;
 }
 
-template <int MODE, bool DIP, bool PAIR>
+// a scalar float's bits other than the sign, tested on the scalar unit (the
+// barrier keeps the compiler from turning it into a float compare, which
+// only the vector unit has)
+__device__ inline bool nonzero_bits(uint32_t b)
+{
+    asm volatile("" : "+s"(b));
+    return (b << 1) != 0u;
+}
+
+// one visit of a staged range: its leaves in `rem & lm`, then its cells in
+// `rem & ~lm`, youngest first, until some lanes open a cell.  Returns that
+// cell's index (with its range and the opening lanes in next_*) and leaves
+// the cells still to visit in rem; -1 when the range is done.  hqm marks
+// the children holding charge (the reference skips the others, pot.c:1732).
+template <bool D, bool CNT>
+__device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_t& rem, uint32_t lm,
+                                  uint32_t hqm, uint64_t tm, float tx, float ty, float tz,
+                                  uint32_t self, float theta2, float soft2, float& px, float& py,
+                                  float& pz, uint32_t& ncell, uint32_t& nleaf, uint32_t& nopen,
+                                  uint32_t& wcell, uint32_t& wleaf, uint32_t& next_first,
+                                  uint32_t& next_cnt, uint64_t& next_mask)
+{
+    const uint32_t lane = __lane_id();
+    const bool act = (tm >> lane) & 1ull;
+    for (uint32_t m = rem & lm; m; m &= m - 1u) {
+        const uint32_t j = (uint32_t)__builtin_ctz(m);
+        const NodeRec32 r = sr[j];
+        const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        // a target skips its own leaf (pot.c:1787); a leaf at float32
+        // distance zero adds nothing
+        const bool pn = r.part != self, dn = d2 != 0.f;
+        const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
+        const bool use = act && pn && dn;
+        const float ca = use ? r.q * (inv * inv * inv) : 0.f;
+        px = __builtin_fmaf(ca, dx, px);
+        py = __builtin_fmaf(ca, dy, py);
+        pz = __builtin_fmaf(ca, dz, pz);
+        wleaf += (uint32_t)__popcll(tm & ballot(pn) & ballot(dn));
+        if (CNT) nleaf += use ? 1u : 0u;
+    }
+    for (uint32_t c = rem & ~lm; c;) {
+        const uint32_t j = 31u - (uint32_t)__builtin_clz(c);
+        c &= ~(1u << j);
+        const NodeRec32 r = sr[j];
+        const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const bool acc = r.size2 < theta2 * d2;
+        const uint64_t am = ballot(acc);
+        const bool use = act && acc;
+        // lanes not taking the cell get zero coefficients (and nothing
+        // infinite, d2 may be zero there)
+        const float inv = use ? __builtin_amdgcn_rsqf(d2) : 0.f;
+        const float inv2 = inv * inv;
+        const float inv3 = inv2 * inv;
+        if (D) {
+            // DIPOLES (pot.c:1738-1745): q d / r^3 - p / r^3 + 3 (d.p) d / r^5
+            const float4 pd = sd[j];
+            const float rpd2 = __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
+            const float ca = inv3 * __builtin_fmaf(3.f, rpd2, r.q);
+            px = __builtin_fmaf(-inv3, pd.x, __builtin_fmaf(ca, dx, px));
+            py = __builtin_fmaf(-inv3, pd.y, __builtin_fmaf(ca, dy, py));
+            pz = __builtin_fmaf(-inv3, pd.z, __builtin_fmaf(ca, dz, pz));
+        } else {
+            // a cell without net charge adds zero
+            const float ca = r.q * inv3;
+            px = __builtin_fmaf(ca, dx, px);
+            py = __builtin_fmaf(ca, dy, py);
+            pz = __builtin_fmaf(ca, dz, pz);
+        }
+        if ((hqm >> j) & 1u) {
+            wcell += (uint32_t)__popcll(tm & am);
+            if (CNT) ncell += use ? 1u : 0u;
+        }
+        const uint64_t om = tm & ~am;
+        if (om) {
+            if (CNT && ((om >> lane) & 1ull)) nopen++;
+            next_first = rfl(r.first);
+            next_cnt = rfl(r.cnt);
+            next_mask = om;
+            rem = c;
+            return (int)j;
+        }
+    }
+    rem = 0;
+    return -1;
+}
+
+template <int MODE, bool DIP>
 __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec32* __restrict__ rec,
-                                                WalkIO io, uint32_t pair_max)
+                                                WalkIO io)
 {
-    __shared__ uint32_t s_first[NW][STKH];
-    __shared__ uint32_t s_cnt[NW][STKH];
-    __shared__ uint64_t s_mask[NW][STKH];
+    __shared__ uint32_t s_first[NW][STK];
+    __shared__ uint32_t s_cnt[NW][STK];  // cnt | cells still to visit << 8
+    __shared__ uint64_t s_mask[NW][STK];
     __shared__ NodeRec32 s_rec[NW][8];
-    __shared__ float4 s_dip[NW][8];
-    __shared__ float s_tx[NW][WAVE], s_ty[NW][WAVE], s_tz[NW][WAVE];
-    __shared__ uint32_t s_self[NW][WAVE];
-    __shared__ float s_ax[NW][WAVE], s_ay[NW][WAVE], s_az[NW][WAVE];
-    __shared__ uint32_t s_packed[NW][WAVE];  // cell | leaf << 10 | open << 20
-    __shared__ uint8_t s_lane[NW][WAVE];
-    __shared__ unsigned long long s_omask[NW][8];
+    __shared__ float4 s_dip[NW][DIP ? 8 : 1];
     const int wv = threadIdx.x / WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    // per-lane counts: the point queries only
+    constexpr bool CNT = MODE == MODE_POINTS;
     unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
 
     for (;;) {
@@ -426,12 +511,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         const float tz = (float)((dtz - p.R) * p.invR);
         const float theta2 = (float)p.theta2;
         const float soft2 = (float)(p.soft2 * p.invR * p.invR);
-        s_tx[wv][lane] = tx;
-        s_ty[wv][lane] = ty;
-        s_tz[wv][lane] = tz;
-        s_self[wv][lane] = self;
         float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         uint32_t ncell = 0, nleaf = 0, nopen = 0;
+        // the tile's totals as scalars (a tile's counts fit 32 bits)
+        uint32_t wcell = 0, wleaf = 0, wopen = 0;
 
         uint64_t pending = ballot(valid);
         while (pending) {
@@ -440,16 +523,20 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
             const uint64_t grp = rfl64(ballot(valid && inst == gi));
             pending &= ~grp;
             const bool dip = DIP && P[gi].dipoles && io.dip32;
-            uint32_t tf = gi, tc = 1;
+            // an empty instance's root has no children and no charge
+            if (rfl(rec[gi].cnt) == 0u) continue;
+            uint32_t tf = gi, tc = 1, rem = 1;
             uint64_t tm = grp;
             int sp = 0;
             uint32_t loads = 0;
             for (;;) {
-                if (tc == 0) {
+                if (rem == 0) {
                     if (sp == 0) break;
                     --sp;
                     tf = rfl(s_first[wv][sp]);
-                    tc = rfl(s_cnt[wv][sp]);
+                    const uint32_t cr = rfl(s_cnt[wv][sp]);
+                    tc = cr & 0xffu;
+                    rem = cr >> 8;
                     tm = rfl64(s_mask[wv][sp]);
                 }
                 if (++loads > io.n_rec || tc > 8) {
@@ -457,141 +544,77 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     break;
                 }
                 tot_pops++;
-                if (lane < tc) {
-                    s_rec[wv][lane] = rec[tf + lane];
-                    if (dip) s_dip[wv][lane] = io.dip32[tf + lane];
+                uint32_t lm = 0, hqm = 0;
+                {
+                    // lane j holds child j: one load for the whole range
+                    NodeRec32 mr;
+                    if (lane < tc) mr = rec[tf + lane];
+                    else mr = NodeRec32{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u, 0u};
+                    if (lane < tc) {
+                        s_rec[wv][lane] = mr;
+                        if (DIP && dip) s_dip[wv][lane] = io.dip32[tf + lane];
+                    }
+                    lm = (uint32_t)ballot(lane < tc && mr.cnt == LEAF);
+                    // DIPOLES takes every cell holding charge, net or not
+                    // (pot.c:1732-1736); internal records carry |q| in `part`
+                    hqm = (uint32_t)ballot(lane < tc && ((DIP && dip) ? mr.part != 0u : mr.q != 0.f));
                 }
-                // the wave reads what its own lanes just wrote: LDS keeps a
-                // wave's accesses in order; keep the compiler from moving them
                 __builtin_amdgcn_wave_barrier();
-                const uint32_t m = __popcll(tm);
-                // a pair step pushes up to tc ranges; child steps push at most
-                // one per level below, so MAXD + 1 entries stay in reserve
-                if (PAIR && m <= pair_max && sp + (int)tc + MAXD + 1 <= STKH) {
-                    const uint32_t lsh = tc > 4 ? 3 : (tc > 2 ? 2 : (tc > 1 ? 1 : 0));
-                    if ((tm >> lane) & 1ull)
-                        s_lane[wv][__popcll(tm & ((1ull << lane) - 1ull))] = (uint8_t)lane;
-                    s_ax[wv][lane] = 0.f;
-                    s_ay[wv][lane] = 0.f;
-                    s_az[wv][lane] = 0.f;
-                    s_packed[wv][lane] = 0u;
-                    if (lane < 8) s_omask[wv][lane] = 0ull;
-                    __builtin_amdgcn_wave_barrier();
-                    const uint32_t total = m << lsh;
-                    for (uint32_t base = 0; base < total; base += WAVE) {
-                        const uint32_t pi = base + lane;
-                        const uint32_t c = pi & ((1u << lsh) - 1u);
-                        if (pi < total && c < tc) {
-                            const uint32_t t = s_lane[wv][pi >> lsh];
-                            float ax = 0.f, ay = 0.f, az = 0.f;
-                            uint32_t cc = 0, cl = 0;
-                            const float4 pd = dip ? s_dip[wv][c] : make_float4(0.f, 0.f, 0.f, 0.f);
-                            const bool op = interact32(s_tx[wv][t], s_ty[wv][t], s_tz[wv][t],
-                                                       s_self[wv][t], theta2, soft2, s_rec[wv][c],
-                                                       dip, pd, ax, ay, az, cc, cl);
-                            if (cc | cl) {
-                                atomicAdd(&s_ax[wv][t], ax);
-                                atomicAdd(&s_ay[wv][t], ay);
-                                atomicAdd(&s_az[wv][t], az);
-                            }
-                            if (op) atomicOr(&s_omask[wv][c], 1ull << t);
-                            const uint32_t pk = cc | (cl << 10) | ((op ? 1u : 0u) << 20);
-                            if (pk) atomicAdd(&s_packed[wv][t], pk);
-                        }
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    const uint32_t pk = s_packed[wv][lane];
-                    ncell += pk & 1023u;
-                    nleaf += (pk >> 10) & 1023u;
-                    nopen += pk >> 20;
-                    kahan_add(fx, kx, s_ax[wv][lane]);
-                    kahan_add(fy, ky, s_ay[wv][lane]);
-                    kahan_add(fz, kz, s_az[wv][lane]);
-                    // push the opened children, the youngest last (visited first)
-                    bool fail = false;
-                    for (uint32_t c = 0; c < tc; c++) {
-                        const uint64_t om = rfl64(s_omask[wv][c]);
-                        if (!om) continue;
-                        const uint32_t cf = rfl(s_rec[wv][c].first), cn = rfl(s_rec[wv][c].cnt);
-                        if (cf <= tf + c || cf + cn > io.n_rec) {
-                            raise_err(g, KERR_CAPACITY, tf + c);
-                            fail = true;
-                            break;
-                        }
-                        s_first[wv][sp] = cf;
-                        s_cnt[wv][sp] = cn;
-                        s_mask[wv][sp] = om;
-                        sp++;
-                    }
-                    if (fail) break;
-                    tc = 0;
-                    continue;
-                }
-                const bool act = (tm >> lane) & 1ull;
                 float px = 0.f, py = 0.f, pz = 0.f;  // this range visit's float32 partial
                 uint32_t next_first = 0, next_cnt = 0;
                 uint64_t next_mask = 0;
-                int j = (int)tc - 1;
-                for (; j >= 0; j--) {
-                    const NodeRec32 r = s_rec[wv][j];
-                    const float4 pd = dip ? s_dip[wv][j] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    bool op = false;
-                    if (act) op = interact32(tx, ty, tz, self, theta2, soft2, r, dip, pd, px, py,
-                                             pz, ncell, nleaf);
-                    const uint64_t om = rfl64(ballot(op));
-                    if (om) {
-                        if (op) nopen++;
-                        next_first = rfl(r.first);
-                        next_cnt = rfl(r.cnt);
-                        next_mask = om;
-                        break;
-                    }
-                }
+                const int j =
+                    (DIP && dip)
+                        ? visit_range<true, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
+                                                 tz, self, theta2, soft2, px, py, pz, ncell, nleaf,
+                                                 nopen, wcell, wleaf, next_first, next_cnt,
+                                                 next_mask)
+                        : visit_range<false, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
+                                                  tz, self, theta2, soft2, px, py, pz, ncell,
+                                                  nleaf, nopen, wcell, wleaf, next_first, next_cnt,
+                                                  next_mask);
                 kahan_add(fx, kx, px);
                 kahan_add(fy, ky, py);
                 kahan_add(fz, kz, pz);
-                if (j < 0) {
-                    tc = 0;
-                    continue;
-                }
-                if (next_first <= tf + (uint32_t)j || next_first + next_cnt > io.n_rec) {
+                if (j < 0) continue;
+                wopen += (uint32_t)__popcll(next_mask);
+                if (next_first <= tf + (uint32_t)j || next_cnt - 1u > 7u ||
+                    next_first + next_cnt > io.n_rec) {
                     raise_err(g, KERR_CAPACITY, tf + j);
                     break;
                 }
-                if (j > 0) {
-                    if (sp >= STKH) {
+                if (rem) {
+                    if (sp >= STK) {
                         raise_err(g, KERR_DEPTH, tf);
                         break;
                     }
                     s_first[wv][sp] = tf;
-                    s_cnt[wv][sp] = (uint32_t)j;
+                    s_cnt[wv][sp] = tc | (rem << 8);
                     s_mask[wv][sp] = tm;
                     sp++;
                 }
                 tf = next_first;
                 tc = next_cnt;
+                rem = (1u << tc) - 1u;
                 tm = next_mask;
             }
         }
         if (valid) {
             const double K = QE / (K4PE0 * p.R * p.R);
-            io.out[3 * k] += K * (double)fx;
-            io.out[3 * k + 1] += K * (double)fy;
-            io.out[3 * k + 2] += K * (double)fz;
+            io.out[3 * k] += K * ((double)fx - (double)kx);
+            io.out[3 * k + 1] += K * ((double)fy - (double)ky);
+            io.out[3 * k + 2] += K * ((double)fz - (double)kz);
             if (io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
                 io.counts[3 * k + 2] = nopen;
             }
         }
-        tot_cell += ncell;
-        tot_leaf += nleaf;
-        tot_open += nopen;
-    }
-    for (int d = 1; d < WAVE; d <<= 1) {
-        tot_cell += __shfl_xor(tot_cell, d, WAVE);
-        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
-        tot_open += __shfl_xor(tot_open, d, WAVE);
+        if (lane == 0) {
+            tot_cell += wcell;
+            tot_leaf += wleaf;
+            tot_open += wopen;
+        }
     }
     if (lane == 0) {
         atomicAdd(&g->n_cell, tot_cell);
@@ -1109,7 +1132,7 @@ template <int MODE>
 static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
 {
     const bool dip = b.dip32 != nullptr;
-#define POTG_W32(KERNEL, ...) KERNEL<<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io __VA_ARGS__)
+#define POTG_W32(KERNEL) KERNEL<<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io)
     if (b.precision == POTG_MIXED_GROUP) {
         if (dip) POTG_W32((k_walk32grp<MODE, true>));
         else POTG_W32((k_walk32grp<MODE, false>));
@@ -1119,12 +1142,9 @@ static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
     } else if (b.group == 16) {
         if (dip) POTG_W32((k_walkg<16, MODE, true>));
         else POTG_W32((k_walkg<16, MODE, false>));
-    } else if (b.pair_max) {
-        if (dip) POTG_W32((k_walk32<MODE, true, true>), , b.pair_max);
-        else POTG_W32((k_walk32<MODE, false, true>), , b.pair_max);
     } else {
-        if (dip) POTG_W32((k_walk32<MODE, true, false>), , 0u);
-        else POTG_W32((k_walk32<MODE, false, false>), , 0u);
+        if (dip) POTG_W32((k_walk32<MODE, true>));
+        else POTG_W32((k_walk32<MODE, false>));
     }
 #undef POTG_W32
 }

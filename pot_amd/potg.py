@@ -59,8 +59,8 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("pair_max", C.c_uint32), ("walk_blocks", C.c_uint32), ("group", C.c_uint32),
-                ("pad", C.c_uint32), ("walk_ms", C.c_double)]
+    _fields_ = [("walk_blocks", C.c_uint32), ("group", C.c_uint32),
+                ("pad", C.c_uint32 * 2), ("walk_ms", C.c_double)]
 
 
 class PotgError(RuntimeError):
@@ -243,18 +243,16 @@ class Context:
     def tune(self):
         t = Tuning()
         _check(self.L.potg_tune(self.h, C.byref(t)))
-        return {"pair_max": t.pair_max, "walk_blocks": t.walk_blocks, "group": t.group,
-                "walk_ms": t.walk_ms}
+        return {"walk_blocks": t.walk_blocks, "group": t.group, "walk_ms": t.walk_ms}
 
     def tuning(self):
         t = Tuning()
         _check(self.L.potg_set_tuning(self.h, None, C.byref(t)))
-        return {"pair_max": t.pair_max, "walk_blocks": t.walk_blocks, "group": t.group,
-                "walk_ms": t.walk_ms}
+        return {"walk_blocks": t.walk_blocks, "group": t.group, "walk_ms": t.walk_ms}
 
-    def set_tuning(self, pair_max, walk_blocks, group=64):
+    def set_tuning(self, walk_blocks, group=64):
         t = Tuning()
-        t.pair_max, t.walk_blocks, t.group = pair_max, walk_blocks, group
+        t.walk_blocks, t.group = walk_blocks, group
         _check(self.L.potg_set_tuning(self.h, C.byref(t), None))
 
     def stats(self):

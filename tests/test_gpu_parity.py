@@ -188,12 +188,12 @@ def test_tuning_takes_no_step_and_is_reproducible():
         b.step()
         before = a.stats()
         t = a.tune()
-        assert 0 <= t["pair_max"] <= 64 and t["walk_blocks"] > 0 and t["walk_ms"] > 0
+        assert t["walk_blocks"] > 0 and t["walk_ms"] > 0
         assert t["group"] in (8, 16, 64)
         after = a.stats()
         for k in ("steps", "n_cell", "n_leaf", "n_open", "n_range_loads"):
             assert after[k] == before[k], k
-        b.set_tuning(t["pair_max"], t["walk_blocks"], t["group"])
+        b.set_tuning(t["walk_blocks"], t["group"])
         for _ in range(3):
             a.step()
             b.step()
@@ -204,11 +204,10 @@ def test_tuning_takes_no_step_and_is_reproducible():
         assert a.stats()["steps"] == 4
 
 
-@pytest.mark.parametrize("group,pair_max", [(64, 0), (64, 1), (64, 7), (64, 64), (8, 0),
-                                            (16, 0)])
+@pytest.mark.parametrize("group,blocks", [(64, 2048), (64, 300), (8, 2048), (16, 2048)])
 @pytest.mark.parametrize("case", [BIG[0], BIG[3]], ids=["monopole", "dipoles"])
-def test_mixed_walk_shapes_agree(case, group, pair_max):
-    """Every walk shape (group width, pair threshold) gives the same
+def test_mixed_walk_shapes_agree(case, group, blocks):
+    """Every walk shape (group width, grid size) gives the same
     interaction sets as the float64 walk (counts equal but for MAC decisions
     float32 flips at the threshold) and the same field to float32 accuracy."""
     sim = oracle_sim(case)
@@ -216,7 +215,7 @@ def test_mixed_walk_shapes_agree(case, group, pair_max):
     ctx64 = gpu_ctx(case, parts)
     ctx64.build_tree()
     ctx = gpu_ctx(case, parts, precision=G.POTG_MIXED)
-    ctx.set_tuning(pair_max, 2048, group)
+    ctx.set_tuning(blocks, group)
     ctx.build_tree()
     idx = np.arange(1, len(parts), 3, dtype=np.uint32)
     xyz = np.stack([parts["x"][idx], parts["y"][idx], parts["z"][idx]], 1)
