@@ -47,6 +47,13 @@ extern "C" {
 #define POTG_PROLATE 1
 #define POTG_OBLATE 2
 
+/* particle integrator: Boris (BORIS, pot.c:130-132), velocity Verlet (the
+   reference without BORIS) or Euler-Richardson (EULER_RICHARDSON,
+   pot.c:122-128); the last two take a second field per step */
+#define POTG_BORIS 0
+#define POTG_VERLET 1
+#define POTG_EULER_RICHARDSON 2
+
 /* arithmetic of the field walk */
 #define POTG_FP64 0  /* float64 everywhere */
 #define POTG_MIXED 1 /* float32 walk with compensated sums; float64 push */
@@ -87,7 +94,14 @@ typedef struct potg_config {
     int32_t sceptic;       /* SCEPTIC_REINJECTION (pot.c:180-182); must be 1 */
     uint64_t seed;         /* key of the counter-based reinjection stream */
     int32_t dipoles;       /* DIPOLES (pot.c:99-105): dipole moments and terms */
+    int32_t integrator;    /* POTG_BORIS, POTG_VERLET or POTG_EULER_RICHARDSON
+                              (pot.c:122-132); one value for every instance
+                              of a context */
+    int32_t out_bounce;    /* OUT_BOUNCE (pot.c:113-120): particles leaving the
+                              domain bounce off its wall instead of being
+                              reinjected */
     int32_t pad;
+    double bounce_coe;     /* OUT_BOUNCE's value, the coefficient of restitution */
 } potg_config;
 
 /* Scalar state of an instance besides its particles. */
@@ -108,7 +122,9 @@ typedef struct potg_step_out {
     uint32_t n_pushed;    /* particles stepped */
     uint32_t n_reinj_abs; /* reinjections after absorption */
     uint32_t n_reinj_out; /* reinjections after leaving the domain */
-    uint32_t pad[2];
+    uint32_t n_wall_inward; /* OUT_BOUNCE: outside the wall but moving inwards,
+                               left alone (pot.c:2724-2728 warns of each) */
+    uint32_t pad;
 } potg_step_out;
 
 /* Work counters accumulated since the last potg_stats reset. */

@@ -39,21 +39,24 @@ SPH_DEL=(-e '/^#define GLFW_INCLUDE_GLU/d' -e '/^#include <GLFW\/glfw.h>/d'
          -e '/^#include <GL\/glu.h>/d' -e '/^#define GL$/d' -e '/^#define OAP 1.0$/d'
          -e '/^#define EXTERNAL_B_[XYZ] 0.0$/d' -e '/^#define RANDOMNESS_SOURCE/d')
 
-build_pot() {  # name theta bx by bz [extra -D...]
+build_pot() {  # name theta bx by bz [extra -D...]; NO_BORIS=1: velocity Verlet
     local name=$1 theta=$2 bx=$3 by=$4 bz=$5
     shift 5
-    sed "${POT_DEL[@]}" "$REF/pot.c" |
+    local del=("${POT_DEL[@]}")
+    if [ -n "${NO_BORIS:-}" ]; then del+=(-e '/^#define BORIS$/d'); fi
+    sed "${del[@]}" "$REF/pot.c" |
         MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
             -DSPHERICAL -DSCEPTIC_REINJECTION -DOAP="$theta" \
             -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
             -DRANDOMNESS_SOURCE="${SEED_SOURCE:-\"seed.bin\"}" "$@" -o "$OUT/$name" -lm
 }
 
-build_sph() {  # name shape theta bx by bz
+build_sph() {  # name shape theta bx by bz; NO_BORIS=1: velocity Verlet
     local name=$1 shape=$2 theta=$3 bx=$4 by=$5 bz=$6
     # the shipped file defines PROLATE (SPH:113); OBLATE replaces it
     local extra=() shape_def=()
     if [ "$shape" = OBLATE ]; then extra=(-e '/^#define PROLATE$/d'); shape_def=(-DOBLATE); fi
+    if [ -n "${NO_BORIS:-}" ]; then extra+=(-e '/^#define BORIS$/d'); fi
     tr -d '\r' < "$REF/pot-spheroid.c" | sed "${SPH_DEL[@]}" "${extra[@]}" |
         MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
             "${shape_def[@]}" -DOAP="$theta" \
@@ -68,6 +71,11 @@ build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
 build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
 # dipole moments and the dipole walk (DIPOLES, pot.c:99-105)
 build_pot pot_sphere_dip 0.5 0.0 0.0 20.0 -DDIPOLES
+# the other integrators and containment (pot.c:113-132): velocity Verlet
+# (BORIS undefined), Euler-Richardson, bouncing off the wall (OUT_BOUNCE)
+NO_BORIS=1 build_pot pot_sphere_verlet_bz 0.5 0.0 0.0 20.0
+NO_BORIS=1 build_pot pot_sphere_er_bounce_bx 0.5 30.0 0.0 0.0 -DEULER_RICHARDSON -DOUT_BOUNCE=0.95
+build_pot pot_sphere_bounce 0.5 0.0 0.0 0.0 -DOUT_BOUNCE=0.95
 # benchmark workloads (pot_amd/workloads.py): sphere_2m_bz (B along z,
 # r_ge = lambda_D / 2) and prolate_4m_tilted_b (B 30 degrees off the axis)
 build_pot pot_sphere_2m_bz 0.5 0.0 0.0 3.72363
@@ -80,5 +88,6 @@ SEED_SOURCE='"/dev/urandom"' build_pot pot_sphere_2m_bz_mpi 0.5 0.0 0.0 3.72363
 build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
 build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0
 build_sph pot_prolate_4m PROLATE 0.5 1.61238 0.0 0.93091
+NO_BORIS=1 build_sph pot_prolate_verlet PROLATE 0.5 0.0 0.0 0.0
 
 echo "build_ref: built $(ls "$OUT" | wc -l) reference variants in $OUT"

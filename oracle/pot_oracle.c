@@ -1404,6 +1404,48 @@ void or_reinject(or_sim* s, or_particle* p, uint32_t ref_idx)
 /* ------------------------------------------------------------------ */
 /* move_particles, pot.c:2330-2934, for a single worker with the BORIS
    integrator (the configurations of this build) */
+/* OUT_BOUNCE containment of a spherical domain, check_particle_in_sim_area
+   (pot.c:2036-2069): outside the wall and moving outwards, the particle's
+   velocity is reflected in spherical components (radial by -coe, the others
+   by coe) and it goes back to where it started the step; outside but moving
+   inwards it is left alone and -1 reported (pot.c:2724-2728).  The angles
+   are computed in long double and kept as double, as the reference's r[]. */
+static int bounce_off_wall(const or_config* ru, or_particle* part, double prev_x, double prev_y,
+                           double prev_z)
+{
+    const double rad = ru->r * ru->mpp;
+    double r[3], v[3];
+    r[0] = pythag(part->x - rad, part->y - rad, part->z - rad);
+    if (r[0] > rad) {
+        r[1] = atan2l(part->y - rad, part->x - rad);
+        r[2] = acosl((part->z - rad) / r[0]);
+        v[0] = part->vx * cos(r[1]) * sin(r[2]) + part->vy * sin(r[1]) * sin(r[2]) +
+               part->vz * cos(r[2]);
+        v[1] = -part->vx * sin(r[1]) + part->vy * cos(r[1]);
+        v[2] = part->vx * cos(r[1]) * cos(r[2]) + part->vy * sin(r[1]) * cos(r[2]) -
+               part->vz * sin(r[2]);
+        if (v[0] > 0.0) {
+            v[0] *= -ru->bounce_coe;
+            v[1] *= ru->bounce_coe;
+            v[2] *= ru->bounce_coe;
+            velocity_in_cartesian_coords(v, r[1], r[2], &part->vx, &part->vy, &part->vz);
+        } else {
+            return -1;
+        }
+        part->x = prev_x;
+        part->y = prev_y;
+        part->z = prev_z;
+    }
+    return 0;
+}
+
+/* move_particles, pot.c:2330-2934 (P = 1: the gathers, the intermediate
+   Bcast and the reductions are identities).  Boris (pot.c:2472-2531) takes
+   one field per particle at its half-drifted position; velocity Verlet and
+   Euler-Richardson (pot.c:2532-2554, 2598-2639) take one at the start
+   position, write the intermediate state to the live array, and take a
+   second there, with the tree built at the start of the step (its cells'
+   moments unchanged, its leaves read live). */
 int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
 {
     const or_config* ru = &s->ru;
@@ -1412,11 +1454,12 @@ int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
     const double B[3] = {ru->B[0], ru->B[1], ru->B[2]};
     const double B_mag = pythag(B[0], B[1], B[2]);
     const double b[3] = {B[0] / B_mag, B[1] / B_mag, B[2] / B_mag};
+    const int boris = ru->integrator == OR_BORIS;
     double dephi;
-    long double dt, E[3], qom;
+    long double dt, E[3], qom, a_x2, a_y2, a_z2;
     unsigned int i, ims, start;
     or_particle *p, *p_n;
-    unsigned int absorbed_e = 0, absorbed_i = 0;
+    unsigned int absorbed_e = 0, absorbed_i = 0, n_wall_inward = 0;
     long double absorbed_m = 0.0;
     int absorbed_q = 0;
     double delta_pos[3];
@@ -1447,11 +1490,13 @@ int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
     for (i = start; i <= last; i++) {
         p = &s->parts[i];
         dt = ru->dt * s->tss[p->species];
-        /* half drift in place; with inplace = 0 the leaves keep reading
-           the build snapshot, so only the target moves */
-        p->x += 0.5 * dt * p->vx;
-        p->y += 0.5 * dt * p->vy;
-        p->z += 0.5 * dt * p->vz;
+        if (boris) {
+            /* half drift in place; with inplace = 0 the leaves keep
+               reading the build snapshot, so only the target moves */
+            p->x += 0.5 * dt * p->vx;
+            p->y += 0.5 * dt * p->vy;
+            p->z += 0.5 * dt * p->vz;
+        }
         field_walk(s, i, p->x, p->y, p->z, src, E, NULL);
         ims = i - start;
         if (p->species == OR_DUST)
@@ -1459,29 +1504,80 @@ int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
         else
             qom = QE * CHARGE[p->species];
         qom /= s->mass[p->species];
-        s->a_x1[ims] = 0.5 * qom * E[0];
-        s->a_y1[ims] = 0.5 * qom * E[1];
-        s->a_z1[ims] = 0.5 * qom * E[2];
-        s->new_parts[i].vx += dt * s->a_x1[ims];
-        s->new_parts[i].vy += dt * s->a_y1[ims];
-        s->new_parts[i].vz += dt * s->a_z1[ims];
-        if (B_mag) {
-            dephi = 2.0 * atanl(0.5 * dt * qom * B_mag);
-            rotate_vec_about_dir(b, sin(dephi), cos(dephi), &s->new_parts[i].vx,
-                                 &s->new_parts[i].vy, &s->new_parts[i].vz, 0.0);
+        if (boris) {
+            s->a_x1[ims] = 0.5 * qom * E[0];
+            s->a_y1[ims] = 0.5 * qom * E[1];
+            s->a_z1[ims] = 0.5 * qom * E[2];
+            s->new_parts[i].vx += dt * s->a_x1[ims];
+            s->new_parts[i].vy += dt * s->a_y1[ims];
+            s->new_parts[i].vz += dt * s->a_z1[ims];
+            if (B_mag) {
+                dephi = 2.0 * atanl(0.5 * dt * qom * B_mag);
+                rotate_vec_about_dir(b, sin(dephi), cos(dephi), &s->new_parts[i].vx,
+                                     &s->new_parts[i].vy, &s->new_parts[i].vz, 0.0);
+            }
+            s->new_parts[i].vx += dt * s->a_x1[ims];
+            s->new_parts[i].vy += dt * s->a_y1[ims];
+            s->new_parts[i].vz += dt * s->a_z1[ims];
+            s->new_parts[i].x = p->x + (0.5 * dt * s->new_parts[i].vx);
+            s->new_parts[i].y = p->y + (0.5 * dt * s->new_parts[i].vy);
+            s->new_parts[i].z = p->z + (0.5 * dt * s->new_parts[i].vz);
+        } else {
+            s->a_x1[ims] = qom * (E[0] + p->vy * B[2] - p->vz * B[1]);
+            s->a_y1[ims] = qom * (E[1] + p->vz * B[0] - p->vx * B[2]);
+            s->a_z1[ims] = qom * (E[2] + p->vx * B[1] - p->vy * B[0]);
+            s->new_parts[i].vx += 0.5 * dt * s->a_x1[ims];
+            s->new_parts[i].vy += 0.5 * dt * s->a_y1[ims];
+            s->new_parts[i].vz += 0.5 * dt * s->a_z1[ims];
+            if (ru->integrator == OR_EULER_RICHARDSON) {
+                s->new_parts[i].x += 0.5 * dt * p->vx;
+                s->new_parts[i].y += 0.5 * dt * p->vy;
+                s->new_parts[i].z += 0.5 * dt * p->vz;
+            } else {
+                s->new_parts[i].x += dt * (p->vx + (0.5 * dt * s->a_x1[ims]));
+                s->new_parts[i].y += dt * (p->vy + (0.5 * dt * s->a_y1[ims]));
+                s->new_parts[i].z += dt * (p->vz + (0.5 * dt * s->a_z1[ims]));
+            }
         }
-        s->new_parts[i].vx += dt * s->a_x1[ims];
-        s->new_parts[i].vy += dt * s->a_y1[ims];
-        s->new_parts[i].vz += dt * s->a_z1[ims];
-        s->new_parts[i].x = p->x + (0.5 * dt * s->new_parts[i].vx);
-        s->new_parts[i].y = p->y + (0.5 * dt * s->new_parts[i].vy);
-        s->new_parts[i].z = p->z + (0.5 * dt * s->new_parts[i].vz);
+    }
+
+    if (!boris) {
+        /* the intermediate state becomes the live array (pot.c:2557-2591);
+           the leaves the second walk reads follow it in both semantics */
+        memcpy(s->parts, s->new_parts, sizeof(or_particle) * ru->N);
+        if (!ru->inplace)
+            memcpy(s->tree_parts, s->parts, sizeof(or_particle) * ru->N);
     }
 
     memset(&g, 0, sizeof(g));
     for (i = start; i <= last; i++) {
+        p = &s->parts[i];
         p_n = &s->new_parts[i];
         ims = i - start;
+        if (!boris) {
+            field_walk(s, i, p->x, p->y, p->z, src, E, NULL);
+            if (p->species == OR_DUST)
+                qom = QE * s->dust_grain_q;
+            else
+                qom = QE * CHARGE[p->species];
+            qom /= s->mass[p->species];
+            a_x2 = qom * (E[0] + p->vy * B[2] - p->vz * B[1]);
+            a_y2 = qom * (E[1] + p->vz * B[0] - p->vx * B[2]);
+            a_z2 = qom * (E[2] + p->vx * B[1] - p->vy * B[0]);
+            dt = ru->dt * s->tss[p->species];
+            if (ru->integrator == OR_EULER_RICHARDSON) {
+                p_n->x = (p->x - 0.5 * dt * (p->vx - 0.5 * dt * s->a_x1[ims])) + dt * p->vx;
+                p_n->y = (p->y - 0.5 * dt * (p->vy - 0.5 * dt * s->a_y1[ims])) + dt * p->vy;
+                p_n->z = (p->z - 0.5 * dt * (p->vz - 0.5 * dt * s->a_z1[ims])) + dt * p->vz;
+                p_n->vx = (p->vx - 0.5 * dt * s->a_x1[ims]) + dt * a_x2;
+                p_n->vy = (p->vy - 0.5 * dt * s->a_y1[ims]) + dt * a_y2;
+                p_n->vz = (p->vz - 0.5 * dt * s->a_z1[ims]) + dt * a_z2;
+            } else {
+                p_n->vx += 0.5 * dt * a_x2;
+                p_n->vy += 0.5 * dt * a_y2;
+                p_n->vz += 0.5 * dt * a_z2;
+            }
+        }
         if (ru->rng_kind == OR_RNG_PHILOX) {
             g.kind = 1;
             pstream_init(&g.ps, ru->philox_seed, i, time_steps);
@@ -1507,7 +1603,10 @@ int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
                 n_reinj_abs++;
             }
         }
-        {
+        if (ru->out_bounce) {
+            if (bounce_off_wall(ru, p_n, s->prev_x[ims], s->prev_y[ims], s->prev_z[ims]) < 0)
+                n_wall_inward++;
+        } else {
             double rad = ru->r * ru->mpp;
             if (pythag(p_n->x - rad, p_n->y - rad, p_n->z - rad) > rad) {
                 inject_g(s, p_n, 1, &g);
@@ -1550,6 +1649,7 @@ int or_move_particles(or_sim* s, int compute_epe, or_step_out* out)
         out->n_reinj_abs = n_reinj_abs;
         out->n_reinj_out = n_reinj_out;
         out->start = start;
+        out->n_wall_inward = n_wall_inward;
     }
     return 0;
 }

@@ -28,6 +28,9 @@ enum { OR_SPHERE = 0, OR_PROLATE = 1, OR_OBLATE = 2 };
 /* reinjection random stream: the reference WELL512 (prng.c:11-28) or the
    counter-based Philox4x32-10 stream the GPU uses */
 enum { OR_RNG_WELL512 = 0, OR_RNG_PHILOX = 1 };
+/* particle integrator: BORIS (pot.c:130-132), velocity Verlet (the
+   reference's default without BORIS, pot.c:122-128) or EULER_RICHARDSON */
+enum { OR_BORIS = 0, OR_VERLET = 1, OR_EULER_RICHARDSON = 2 };
 
 /* pot.c:295-303: the reference's 56-byte AoS particle record */
 typedef struct or_particle {
@@ -67,6 +70,9 @@ typedef struct or_config {
     uint64_t philox_seed;    /* key of the counter-based stream */
     int32_t record_macro;    /* RECORD_MACRO_DATA switch (drives compute_epe cadence) */
     int32_t dipoles;         /* DIPOLES switch (pot.c:99-105): cell dipole moments */
+    int32_t integrator;      /* OR_BORIS / OR_VERLET / OR_EULER_RICHARDSON */
+    int32_t out_bounce;      /* OUT_BOUNCE switch (pot.c:113-120): bounce off the wall */
+    double bounce_coe;       /* OUT_BOUNCE's value: coefficient of restitution */
 } or_config;
 
 /* Per-walk interaction counters. */
@@ -86,7 +92,8 @@ typedef struct or_step_out {
     uint32_t n_reinj_abs;    /* reinjections after grain absorption */
     uint32_t n_reinj_out;    /* reinjections after leaving the domain */
     uint32_t start;          /* first stepped index */
-    uint32_t pad;
+    uint32_t n_wall_inward;  /* OUT_BOUNCE: outside the wall but moving inwards
+                                (check_particle_in_sim_area returns -1) */
 } or_step_out;
 
 typedef struct or_sim or_sim;

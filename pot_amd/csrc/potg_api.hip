@@ -116,6 +116,9 @@ int validate(const potg_config& k, uint32_t i)
         return set_err("instance %u: prolate aspect ratio must be greater than 1", i), POTG_ERR_ARG;
     if (k.geometry == POTG_OBLATE && !(k.A < 1.0 && k.A > 0.0))
         return set_err("instance %u: oblate aspect ratio must be less than 1", i), POTG_ERR_ARG;
+    if (k.integrator < POTG_BORIS || k.integrator > POTG_EULER_RICHARDSON ||
+        (k.out_bounce && !(k.bounce_coe >= 0.0)))
+        return set_err("instance %u: bad integrator or bounce coefficient", i), POTG_ERR_ARG;
     return POTG_OK;
 }
 
@@ -158,6 +161,9 @@ void fill_params(InstParams& p, const potg_config& k)
     p.sceptic = k.sceptic;
     p.seed = k.seed;
     p.dipoles = k.dipoles != 0;
+    p.integrator = k.integrator;
+    p.out_bounce = k.out_bounce != 0;
+    p.bounce_coe = k.bounce_coe;
     p.N = (uint32_t)k.N;
     for (int s = 0; s < 3; s++) p.first_idx[s] = NONE;
 }
@@ -207,6 +213,15 @@ void enqueue_step(potg_ctx* c, int compute_epe)
     launch_walk_init(b, s);
     record(c, 2);
     launch_walk(b, s);
+    if (b.integrator != POTG_BORIS) {
+        // velocity Verlet / Euler-Richardson: the second field, at the
+        // intermediate state, with the leaves moved there
+        launch_half(b, s);
+        launch_leaf_refresh(b, s);
+        launch_walk_init(b, s);
+        launch_walk_reset(b, s);
+        launch_walk(b, s);
+    }
     if (compute_epe) launch_epe(b, s);
     record(c, 3);
     launch_push(b, s);
@@ -275,6 +290,10 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     for (uint32_t i = 0; i < n_inst; i++) {
         int rc = validate(configs[i], i);
         if (rc) return rc;
+        // the step's launch sequence depends on the integrator
+        if (configs[i].integrator != configs[0].integrator)
+            return set_err("potg_create: instances of one context need one integrator"),
+                   POTG_ERR_ARG;
         n_total += configs[i].N;
         n_tree += configs[i].N - (configs[i].geometry != POTG_SPHERE ? 1 : 0);
     }
@@ -374,6 +393,11 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
     rc |= alloc_n(c, &b.queue, nt);
     rc |= alloc_n(c, &b.E, 3 * nt) | alloc_n(c, &b.inv, n);
+    b.integrator = configs[0].integrator;
+    if (b.integrator != POTG_BORIS) rc |= alloc_n(c, &b.a1, 3 * nt) | alloc_n(c, &b.x0, 3 * nt);
+    bool any_bounce = false;
+    for (uint32_t i = 0; i < n_inst; i++) any_bounce |= configs[i].out_bounce != 0;
+    if (any_bounce) rc |= alloc_n(c, &b.qprev, 3 * nt);
     rc |= alloc_n(c, &b.phi, n) | alloc_n(c, &b.epe_part, 2 * (size_t)EPE_BLOCKS * n_inst);
     b.q_cap = (uint32_t)(n < 65536 ? 65536 : n);
     rc |= alloc_n(c, &b.q_pts, 3 * (size_t)b.q_cap) | alloc_n(c, &b.q_inst, b.q_cap);
@@ -535,6 +559,7 @@ int potg_step(potg_ctx* c, int32_t compute_epe, potg_step_out* out)
         out[i].n_pushed = s[i].out_pushed;
         out[i].n_reinj_abs = s[i].out_reinj_abs;
         out[i].n_reinj_out = s[i].out_reinj_out;
+        out[i].n_wall_inward = s[i].out_wall_inward;
     }
     return POTG_OK;
 }

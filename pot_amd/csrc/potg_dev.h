@@ -69,6 +69,9 @@ struct InstParams {
     uint32_t tree_n;     // tree particles (N, or N - 1 when the grain is analytic)
     uint32_t first_idx[3]; // lowest reference index of each species
     int32_t dipoles;       // DIPOLES: dipole moments in the tree and the walks
+    int32_t integrator;    // POTG_BORIS / POTG_VERLET / POTG_EULER_RICHARDSON
+    int32_t out_bounce;    // OUT_BOUNCE: bounce off the wall
+    double bounce_coe;     // its coefficient of restitution
 };
 
 // Mutable per-instance state on the device.
@@ -83,11 +86,11 @@ struct InstState {
     uint32_t root;       // walk id of the tree root
     // per-step accumulators (cleared by the step's first kernel)
     int32_t acc_q;
-    uint32_t acc_e, acc_i, n_pushed, n_reinj_abs, n_reinj_out;
+    uint32_t acc_e, acc_i, n_pushed, n_reinj_abs, n_reinj_out, n_wall_inward;
     double acc_p[3];
     // outputs of the last step
     int32_t out_q;
-    uint32_t out_e, out_i, out_pushed, out_reinj_abs, out_reinj_out;
+    uint32_t out_e, out_i, out_pushed, out_reinj_abs, out_reinj_out, out_wall_inward;
 };
 
 // Context-wide device counters.

@@ -75,6 +75,13 @@ struct Bufs {
     uint32_t* targets;
     uint32_t* queue;   // reinjection queue
     double* E;
+    // velocity Verlet / Euler-Richardson: the first half's acceleration and
+    // each target's start position; OUT_BOUNCE: the start position of each
+    // queued reinjection
+    int integrator;    // POTG_BORIS / POTG_VERLET / POTG_EULER_RICHARDSON
+    double* a1;
+    double* x0;
+    double* qprev;
     uint32_t* inv;     // global reference slot -> sorted slot
     // potential energies (compute_epe steps)
     double* phi;       // potential felt by each sorted slot
@@ -99,6 +106,10 @@ void launch_walk(Bufs& b, hipStream_t s);
 void launch_walk_reset(Bufs& b, hipStream_t s);  // rewind the walk's tile counter
 int device_cus();
 void launch_push(Bufs& b, hipStream_t s);
+// velocity Verlet / Euler-Richardson: first half of the push (state becomes
+// the intermediate state), then the leaf records follow it
+void launch_half(Bufs& b, hipStream_t s);
+void launch_leaf_refresh(Bufs& b, hipStream_t s);
 // compute_epe nonzero: fold the potential-energy sums of launch_epe
 void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s);
 void launch_epe(Bufs& b, hipStream_t s);

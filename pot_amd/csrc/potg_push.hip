@@ -312,11 +312,90 @@ __device__ bool crosses(const InstParams& p, double px, double py, double pz, co
     return (s2 - (alpha * d_dot_s)) <= (a * a);
 }
 
+// OUT_BOUNCE containment of the spherical domain, check_particle_in_sim_area
+// (pot.c:2036-2069): outside the wall and moving outwards, the velocity is
+// reflected in spherical components (radial by -coe, the others by coe)
+// and the particle goes back to where it started the step; outside and
+// moving inwards it is left alone, and -1 reported (pot.c:2724-2728).
+__device__ int bounce_off_wall(const InstParams& p, double& x, double& y, double& z, double& vx,
+                               double& vy, double& vz, double px, double py, double pz)
+{
+    const double rad = p.R;
+    const double dx = x - rad, dy = y - rad, dz = z - rad;
+    const double r0 = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r0 > rad) {
+        const double ph = atan2(dy, dx);
+        const double th = acos(dz / r0);
+        const double cp = cos(ph), sp = sin(ph), ct = cos(th), st = sin(th);
+        double v[3];
+        v[0] = vx * cp * st + vy * sp * st + vz * ct;
+        v[1] = -vx * sp + vy * cp;
+        v[2] = vx * cp * ct + vy * sp * ct - vz * st;
+        if (!(v[0] > 0.0)) return -1;
+        v[0] *= -p.bounce_coe;
+        v[1] *= p.bounce_coe;
+        v[2] *= p.bounce_coe;
+        vel_cart(v, ph, th, vx, vy, vz);
+        x = px;
+        y = py;
+        z = pz;
+    }
+    return 0;
+}
+
+// Velocity Verlet and Euler-Richardson, first half (pot.c:2532-2554): the
+// acceleration at the start of the step (E and v x B), the velocity half
+// step and the integrator's position update.  The intermediate state
+// replaces the particle's state, as the reference's gather and broadcast
+// replace its live array (pot.c:2557-2591); the acceleration and the start
+// position are kept for the second half.
+__global__ __launch_bounds__(BT) void k_half(const InstParams* __restrict__ P,
+                                              const Globals* __restrict__ g, PState st,
+                                              const uint32_t* __restrict__ targets,
+                                              const double* __restrict__ E, double* __restrict__ a1,
+                                              double* __restrict__ x0)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g->n_targets) return;
+    const uint32_t i = targets[k];
+    const InstParams& p = P[st.inst[i]];
+    const int s = st.spec[i];
+    const double dts = p.dt * (double)p.tss[s];
+    const double hdt = 0.5 * dts;
+    const double qom = (QE * (s == 1 ? 1 : -1)) / p.mass[s];
+    const double x = st.x[i], y = st.y[i], z = st.z[i];
+    const double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
+    const double ax = qom * (E[3 * k] + vy * p.B[2] - vz * p.B[1]);
+    const double ay = qom * (E[3 * k + 1] + vz * p.B[0] - vx * p.B[2]);
+    const double az = qom * (E[3 * k + 2] + vx * p.B[1] - vy * p.B[0]);
+    x0[3 * k] = x;
+    x0[3 * k + 1] = y;
+    x0[3 * k + 2] = z;
+    a1[3 * k] = ax;
+    a1[3 * k + 1] = ay;
+    a1[3 * k + 2] = az;
+    st.vx[i] = vx + hdt * ax;
+    st.vy[i] = vy + hdt * ay;
+    st.vz[i] = vz + hdt * az;
+    if (p.integrator == 2) {
+        st.x[i] = x + hdt * vx;
+        st.y[i] = y + hdt * vy;
+        st.z[i] = z + hdt * vz;
+    } else {
+        st.x[i] = x + dts * (vx + hdt * ax);
+        st.y[i] = y + dts * (vy + hdt * ay);
+        st.z[i] = z + dts * (vz + hdt * az);
+    }
+}
+
 __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, InstState* S,
                                               Globals* g, PState st,
                                               const uint32_t* __restrict__ targets,
                                               const double* __restrict__ E,
-                                              uint32_t* __restrict__ queue, uint32_t qcap)
+                                              const double* __restrict__ a1,
+                                              const double* __restrict__ xs,
+                                              uint32_t* __restrict__ queue,
+                                              double* __restrict__ qprev, uint32_t qcap)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ntg = g->n_targets;
@@ -330,27 +409,66 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
         const uint64_t t = g->t;
         const double dts = p.dt * (double)p.tss[s];
         const double hdt = 0.5 * dts;
-        const double x0 = st.x[i], y0 = st.y[i], z0 = st.z[i];
-        double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
-        const double xh = x0 + hdt * vx, yh = y0 + hdt * vy, zh = z0 + hdt * vz;
         const double qom = (QE * (s == 1 ? 1 : -1)) / p.mass[s];
-        const double ax = 0.5 * qom * E[3 * k], ay = 0.5 * qom * E[3 * k + 1],
-                     az = 0.5 * qom * E[3 * k + 2];
-        vx += dts * ax;
-        vy += dts * ay;
-        vz += dts * az;
-        if (p.Bmag != 0.0) {
-            const double dephi = 2.0 * atan(hdt * qom * p.Bmag);
-            rot_dir(p.b, sin(dephi), cos(dephi), vx, vy, vz);
+        double x0, y0, z0, x, y, z, vx, vy, vz;
+        if (p.integrator == 0) {
+            // Boris (pot.c:2472-2531)
+            x0 = st.x[i];
+            y0 = st.y[i];
+            z0 = st.z[i];
+            vx = st.vx[i];
+            vy = st.vy[i];
+            vz = st.vz[i];
+            const double xh = x0 + hdt * vx, yh = y0 + hdt * vy, zh = z0 + hdt * vz;
+            const double ax = 0.5 * qom * E[3 * k], ay = 0.5 * qom * E[3 * k + 1],
+                         az = 0.5 * qom * E[3 * k + 2];
+            vx += dts * ax;
+            vy += dts * ay;
+            vz += dts * az;
+            if (p.Bmag != 0.0) {
+                const double dephi = 2.0 * atan(hdt * qom * p.Bmag);
+                rot_dir(p.b, sin(dephi), cos(dephi), vx, vy, vz);
+            }
+            vx += dts * ax;
+            vy += dts * ay;
+            vz += dts * az;
+            x = xh + hdt * vx;
+            y = yh + hdt * vy;
+            z = zh + hdt * vz;
+        } else {
+            // second half of velocity Verlet / Euler-Richardson
+            // (pot.c:2598-2639): the field at the intermediate state, which
+            // the state holds, and the first half's acceleration
+            x0 = xs[3 * k];
+            y0 = xs[3 * k + 1];
+            z0 = xs[3 * k + 2];
+            const double xm = st.x[i], ym = st.y[i], zm = st.z[i];
+            const double vhx = st.vx[i], vhy = st.vy[i], vhz = st.vz[i];
+            const double bx = qom * (E[3 * k] + vhy * p.B[2] - vhz * p.B[1]);
+            const double by = qom * (E[3 * k + 1] + vhz * p.B[0] - vhx * p.B[2]);
+            const double bz = qom * (E[3 * k + 2] + vhx * p.B[1] - vhy * p.B[0]);
+            if (p.integrator == 2) {
+                const double ax = a1[3 * k], ay = a1[3 * k + 1], az = a1[3 * k + 2];
+                x = (xm - hdt * (vhx - hdt * ax)) + dts * vhx;
+                y = (ym - hdt * (vhy - hdt * ay)) + dts * vhy;
+                z = (zm - hdt * (vhz - hdt * az)) + dts * vhz;
+                vx = (vhx - hdt * ax) + dts * bx;
+                vy = (vhy - hdt * ay) + dts * by;
+                vz = (vhz - hdt * az) + dts * bz;
+            } else {
+                x = xm;
+                y = ym;
+                z = zm;
+                vx = vhx + hdt * bx;
+                vy = vhy + hdt * by;
+                vz = vhz + hdt * bz;
+            }
         }
-        vx += dts * ax;
-        vy += dts * ay;
-        vz += dts * az;
-        double x = xh + hdt * vx, y = yh + hdt * vy, z = zh + hdt * vz;
 
         // grain collection (pot.c:2641-2685) and the boundary test
-        // (pot.c:2071-2078); the rare reinjections go to a queue so that the
-        // long SCEPTIC sampling runs in lanes of its own (k_reinject)
+        // (pot.c:2071-2078) or the bounce off the wall (pot.c:2036-2069);
+        // the rare reinjections go to a queue so that the long SCEPTIC
+        // sampling runs in lanes of its own (k_reinject)
         int fate = 0;  // 1: absorbed, 2: left the domain
         if (p.grain_enabled && ((double)t * p.dt) > p.settle) {
             const double d[3] = {x - x0, y - y0, z - z0};
@@ -366,16 +484,28 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
             }
         }
         if (!fate) {
-            const double rad = p.R;
-            const double dx = x - rad, dy = y - rad, dz = z - rad;
-            if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
-                atomicAdd(&S[inst].n_reinj_out, 1u);
-                fate = 2;
+            if (p.out_bounce) {
+                if (bounce_off_wall(p, x, y, z, vx, vy, vz, x0, y0, z0) < 0)
+                    atomicAdd(&S[inst].n_wall_inward, 1u);
+            } else {
+                const double rad = p.R;
+                const double dx = x - rad, dy = y - rad, dz = z - rad;
+                if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
+                    atomicAdd(&S[inst].n_reinj_out, 1u);
+                    fate = 2;
+                }
             }
         }
         if (fate) {
             const uint32_t q = atomicAdd(&g->n_reinj, 1u);
-            if (q < qcap) queue[q] = (i << 1) | (fate == 1 ? 1u : 0u);
+            if (q < qcap) {
+                queue[q] = (i << 1) | (fate == 1 ? 1u : 0u);
+                if (qprev) {
+                    qprev[3 * (size_t)q] = x0;
+                    qprev[3 * (size_t)q + 1] = y0;
+                    qprev[3 * (size_t)q + 2] = z0;
+                }
+            }
         }
         st.x[i] = x;
         st.y[i] = y;
@@ -393,7 +523,8 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
 // of the queue does not matter.
 __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ P, InstState* S,
                                                   Globals* g, PState st,
-                                                  const uint32_t* __restrict__ queue, uint32_t qcap)
+                                                  const uint32_t* __restrict__ queue,
+                                                  const double* __restrict__ qprev, uint32_t qcap)
 {
     const uint32_t n = g->n_reinj < qcap ? g->n_reinj : qcap;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
@@ -408,7 +539,13 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
         double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
         if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
         else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
-        if (code & 1u) {
+        if ((code & 1u) && p.out_bounce) {
+            // the absorbed particle's start position is where a bounce
+            // returns it (pot.c:2686-2690)
+            if (bounce_off_wall(p, x, y, z, vx, vy, vz, qprev[3 * (size_t)e],
+                                qprev[3 * (size_t)e + 1], qprev[3 * (size_t)e + 2]) < 0)
+                atomicAdd(&S[inst].n_wall_inward, 1u);
+        } else if (code & 1u) {
             const double rad = p.R;
             const double dx = x - rad, dy = y - rad, dz = z - rad;
             if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
@@ -475,6 +612,7 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
         s.out_pushed = s.start == NONE ? 0u : p.N - s.start;
         s.out_reinj_abs = s.n_reinj_abs;
         s.out_reinj_out = s.n_reinj_out;
+        s.out_wall_inward = s.n_wall_inward;
     }
     if (k == 0) {
         g->n_tgt_total += g->n_targets;
@@ -640,8 +778,14 @@ void launch_targets(Bufs& b, hipStream_t s)
 
 void launch_push(Bufs& b, hipStream_t s)
 {
-    k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.queue, b.n_tree);
-    k_reinject<<<64, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.n_tree);
+    k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.a1, b.x0, b.queue,
+                                         b.qprev, b.n_tree);
+    k_reinject<<<64, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.qprev, b.n_tree);
+}
+
+void launch_half(Bufs& b, hipStream_t s)
+{
+    k_half<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.g, b.st, b.targets, b.E, b.a1, b.x0);
 }
 
 void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s)

@@ -85,6 +85,7 @@ __global__ void k_begin_step(InstParams* __restrict__ P, InstState* __restrict__
         S[k].n_pushed = 0;
         S[k].n_reinj_abs = 0;
         S[k].n_reinj_out = 0;
+        S[k].n_wall_inward = 0;
         S[k].acc_p[0] = 0.0;
         S[k].acc_p[1] = 0.0;
         S[k].acc_p[2] = 0.0;
@@ -703,6 +704,19 @@ __global__ __launch_bounds__(1024) void k_level_moments_block(
 void launch_begin_step(Bufs& b, hipStream_t s)
 {
     k_begin_step<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst);
+}
+
+// the leaf records again, from the current state: between the two fields
+// of velocity Verlet / Euler-Richardson the reference's walk reads its leaves
+// from the live array, which then holds the intermediate state, while the
+// cells keep the moments of the step's build (pot.c:1785-1804, 2557-2613)
+void launch_leaf_refresh(Bufs& b, hipStream_t s)
+{
+    const uint32_t n = b.n_tree;
+    if (!n) return;
+    k_leaf_records<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.n_rec, b.wid_leaf, b.rec64,
+                                          b.precision ? b.rec32 : nullptr, b.dip64,
+                                          b.precision ? b.dip32 : nullptr);
 }
 
 void launch_keys_sort_permute(Bufs& b, hipStream_t s)

@@ -80,7 +80,10 @@ __device__ inline double bcast_d(double v, uint32_t j)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// target of lane k: the half-drifted particle (pot.c:2478-2480) or a point
+// target of lane k: the half-drifted particle (Boris, pot.c:2478-2480), the
+// particle where the state has it (velocity Verlet / Euler-Richardson, whose
+// fields are taken at the start and the intermediate state, pot.c:2483-2485,
+// 2603-2605), or a point
 template <int MODE>
 __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, const WalkIO& io,
                                    uint32_t k, uint32_t ntg, uint32_t& inst, uint32_t& self,
@@ -94,15 +97,24 @@ __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, 
         const uint32_t i = io.targets[k];
         self = i;
         inst = st.inst[i];
-        const double dts = P[inst].dt * (double)P[inst].tss[st.spec[i]];
-        tx = st.x[i] + (0.5 * dts) * st.vx[i];
-        ty = st.y[i] + (0.5 * dts) * st.vy[i];
-        tz = st.z[i] + (0.5 * dts) * st.vz[i];
+        if (P[inst].integrator == 0) {
+            const double dts = P[inst].dt * (double)P[inst].tss[st.spec[i]];
+            tx = st.x[i] + (0.5 * dts) * st.vx[i];
+            ty = st.y[i] + (0.5 * dts) * st.vy[i];
+            tz = st.z[i] + (0.5 * dts) * st.vz[i];
+        } else {
+            tx = st.x[i];
+            ty = st.y[i];
+            tz = st.z[i];
+        }
     } else if (MODE == MODE_EPE) {
         self = k;
         inst = st.inst[k];
         const int sp = st.spec[k];
-        const bool stepped = sp != 0 && st.ref[k] - P[inst].offset >= io.S[inst].start;
+        // Boris half-drifted the stepped particles in place; the other
+        // integrators' state already holds the intermediate state
+        const bool stepped = sp != 0 && P[inst].integrator == 0 &&
+                             st.ref[k] - P[inst].offset >= io.S[inst].start;
         const double hdt = stepped ? 0.5 * (P[inst].dt * (double)P[inst].tss[sp]) : 0.0;
         tx = stepped ? st.x[k] + hdt * st.vx[k] : st.x[k];
         ty = stepped ? st.y[k] + hdt * st.vy[k] : st.y[k];
@@ -1124,10 +1136,10 @@ void launch_walk_init(Bufs& b, hipStream_t s)
     k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
 }
 
-// float32 field walk of the chosen shape: groups of 8 or 16 targets
-// (k_walkg), or 64 targets stepping through the children with or without
-// the pair mode (k_walk32); instantiated with dipole support only for
-// contexts holding DIPOLES instances
+// float32 field walk of the chosen shape: the group criterion
+// (k_walk32grp), groups of 8 or 16 targets (k_walkg), or 64 targets stepping
+// through the children (k_walk32); instantiated with dipole support only
+// for contexts holding DIPOLES instances
 template <int MODE>
 static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
 {

@@ -143,11 +143,17 @@ void pot_write_run_description(const char* path, const pot_run* ru, const pot_ru
     fputs(ru->grain ? "grain:                enabled\n" : "grain:                disabled\n", fp);
     fputs("OpenGL interface:     no\n", fp);
     fputs("trajectory length:    N/A\n", fp);
-    fputs("containment:          reinjection (SCEPTIC-style)\n", fp);
+    if (ru->out_bounce)
+        fprintf(fp, "containment:          bouncing (c.o.e. %g)\n", ru->bounce_coe);
+    else
+        fputs("containment:          reinjection (SCEPTIC-style)\n", fp);
     fputs("FP exceptions:        no\n", fp);
     fputs(ru->dipoles ? "multipole expansion:  dipole\n" : "multipole expansion:  monopole\n", fp);
     fputs("simulation shape:     spherical\n", fp);
-    fputs("numerical integrator: Boris\n", fp);
+    fputs(ru->integrator == POTG_EULER_RICHARDSON ? "numerical integrator: Euler-Richardson\n"
+          : ru->integrator == POTG_VERLET ? "numerical integrator: velocity Verlet\n"
+                                          : "numerical integrator: Boris\n",
+          fp);
     fputs("pos. Pois. collec'n:  no\n", fp);
     fputs("Langmuir test wave:   no\n", fp);
     if (sph) {

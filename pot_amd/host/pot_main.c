@@ -19,6 +19,11 @@
  *   --mime X         MIME, ion/electron mass ratio (pot.c:60)
  *   --no-grain       DISABLE_GRAIN (pot.c:30)
  *   --dipoles        DIPOLES: dipole moments in the tree (pot.c:99-105)
+ *   --verlet         velocity Verlet, the reference without BORIS
+ *                    (pot.c:122-132)
+ *   --euler-richardson  EULER_RICHARDSON (pot.c:122-128)
+ *   --bounce COE     OUT_BOUNCE with coefficient of restitution COE:
+ *                    particles bounce off the wall (pot.c:113-120)
  *   --oblate         OBLATE instead of PROLATE (pot-spheroid only)
  *   --fp64           float64 walk (default: float32 walk with compensated
  *                    sums, float64 push)
@@ -59,6 +64,9 @@ static void set_defaults(pot_run* ru)
     ru->tim_lim = 0.0;
     ru->grain = 1;
     ru->dipoles = 0;
+    ru->integrator = POTG_BORIS;
+    ru->out_bounce = 0;
+    ru->bounce_coe = 0.0;
     ru->precision = POTG_MIXED;
     ru->seed_path = "/dev/urandom";
     ru->out_dir = "./output/";
@@ -105,6 +113,7 @@ static void usage(const char* prog)
           stderr);
     if (SPHEROID_PROGRAM) fputs("  -A [NUMBER]   aspect ratio of the spheroid\n", stderr);
     fputs("  --theta X  --bx X --by X --bz X  --mime X  --no-grain  --dipoles  --fp64\n"
+          "  --verlet  --euler-richardson  --bounce COE\n"
           "  --seed-file PATH  --output-dir DIR", stderr);
     fputs(SPHEROID_PROGRAM ? "  --oblate\n" : "\n", stderr);
 }
@@ -137,6 +146,14 @@ static int parse_args(int argc, char** argv, pot_run* ru)
                 ru->precision = POTG_FP64;
                 continue;
             }
+            if (!strcmp(o, "--verlet")) {
+                ru->integrator = POTG_VERLET;
+                continue;
+            }
+            if (!strcmp(o, "--euler-richardson")) {
+                ru->integrator = POTG_EULER_RICHARDSON;
+                continue;
+            }
             if (!strcmp(o, "--oblate") && SPHEROID_PROGRAM) {
                 ru->geometry = POTG_OBLATE;
                 continue;
@@ -155,6 +172,11 @@ static int parse_args(int argc, char** argv, pot_run* ru)
             } else if (!strcmp(o, "--mime")) {
                 ru->mime = strtod(v, NULL);
                 if (!(ru->mime > 0.0)) return bad("Mass ratio must be positive.");
+            } else if (!strcmp(o, "--bounce")) {
+                ru->out_bounce = 1;
+                ru->bounce_coe = strtod(v, NULL);
+                if (!(ru->bounce_coe >= 0.0))
+                    return bad("Coefficient of restitution must be non-negative.");
             } else if (!strcmp(o, "--seed-file")) {
                 ru->seed_path = v;
             } else if (!strcmp(o, "--output-dir")) {
@@ -390,6 +412,9 @@ static potg_config device_config(const pot_run* ru, uint64_t key)
     c.sceptic = 1;
     c.seed = key;
     c.dipoles = ru->dipoles;
+    c.integrator = ru->integrator;
+    c.out_bounce = ru->out_bounce;
+    c.bounce_coe = ru->bounce_coe;
     return c;
 }
 
@@ -455,6 +480,7 @@ static int run(host_run* h, uint64_t t_completed)
     unsigned long t = (unsigned long)t_completed;
     const time_t start = time(NULL);
     int rec = 0, failed = 0;
+    potg_step_out so;
     char path[4096];
 
     out_path(path, sizeof(path), ru, "pot-run-init-info.txt");
@@ -487,11 +513,16 @@ static int run(host_run* h, uint64_t t_completed)
             pot_write_state(ru->sw_path, ru->N, h->parts, &h->st, 1);
         }
         rec = record_macro(ru, t);
-        if (potg_step(h->ctx, rec, NULL) != POTG_OK || potg_sync(h->ctx) != POTG_OK) {
+        if (potg_step(h->ctx, rec, ru->out_bounce ? &so : NULL) != POTG_OK ||
+            potg_sync(h->ctx) != POTG_OK) {
             fprintf(stderr, "0:%lu: %s\n", t, potg_last_error());
             failed = 1;
             break;
         }
+        /* the reference warns of each such particle (pot.c:2724-2728) */
+        if (ru->out_bounce && so.n_wall_inward)
+            fprintf(stderr, "0: warning: %u particle(s) in outer wall travelling inwards.\n",
+                    so.n_wall_inward);
         if (!(t % 20000)) write_phi(ru, h->ctx, &h->parts[0], t);
         if (rec) {
             if (fetch(h)) {

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libpotg.so")
 
 POTG_SPHERE, POTG_PROLATE, POTG_OBLATE = 0, 1, 2
 POTG_FP64, POTG_MIXED, POTG_MIXED_GROUP = 0, 1, 2
+POTG_BORIS, POTG_VERLET, POTG_EULER_RICHARDSON = 0, 1, 2
 DUST, ION, ELECTRON = 0, 1, 2
 
 PARTICLE_DT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),
@@ -37,7 +38,8 @@ class Config(C.Structure):
                 ("Te", C.c_double), ("Ti", C.c_double), ("A", C.c_double), ("theta", C.c_double),
                 ("B", C.c_double * 3), ("max_lin_dev", C.c_double), ("mime", C.c_double),
                 ("grain_enabled", C.c_int32), ("sceptic", C.c_int32), ("seed", C.c_uint64),
-                ("dipoles", C.c_int32), ("pad", C.c_int32)]
+                ("dipoles", C.c_int32), ("integrator", C.c_int32), ("out_bounce", C.c_int32),
+                ("pad", C.c_int32), ("bounce_coe", C.c_double)]
 
 
 class Grain(C.Structure):
@@ -48,7 +50,7 @@ class Grain(C.Structure):
 class StepOut(C.Structure):
     _fields_ = [("abs_q", C.c_int32), ("abs_e", C.c_uint32), ("abs_i", C.c_uint32),
                 ("n_pushed", C.c_uint32), ("n_reinj_abs", C.c_uint32),
-                ("n_reinj_out", C.c_uint32), ("pad", C.c_uint32 * 2)]
+                ("n_reinj_out", C.c_uint32), ("n_wall_inward", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -123,7 +125,7 @@ def _up(a):
 
 def config(N, r, mpp, dt, soft, a, Te, Ti, x_drift=0.0, settle=0.0, geometry=POTG_SPHERE,
            A=1.0, theta=0.5, B=(0.0, 0.0, 0.0), seed=1, grain_enabled=1, sceptic=1,
-           mime=None, max_lin_dev=None, dipoles=0):
+           mime=None, max_lin_dev=None, dipoles=0, integrator=0, bounce=None):
     c = Config()
     c.N, c.r, c.geometry = int(N), int(r), int(geometry)
     c.dt, c.mpp, c.soft, c.x_drift, c.a, c.settle = dt, mpp, soft, x_drift, a, settle
@@ -134,6 +136,9 @@ def config(N, r, mpp, dt, soft, a, Te, Ti, x_drift=0.0, settle=0.0, geometry=POT
     c.mime = mime if mime is not None else (1836.153 if sph else 1836.15)
     c.grain_enabled, c.sceptic, c.seed = grain_enabled, sceptic, seed
     c.dipoles = dipoles
+    c.integrator = integrator
+    c.out_bounce = 0 if bounce is None else 1
+    c.bounce_coe = 0.0 if bounce is None else bounce
     return c
 
 
@@ -184,7 +189,8 @@ class Context:
         out = (StepOut * self.n_inst)()
         _check(self.L.potg_step(self.h, compute_epe, out))
         return [{"abs_q": o.abs_q, "abs_e": o.abs_e, "abs_i": o.abs_i, "n_pushed": o.n_pushed,
-                 "n_reinj_abs": o.n_reinj_abs, "n_reinj_out": o.n_reinj_out} for o in out]
+                 "n_reinj_abs": o.n_reinj_abs, "n_reinj_out": o.n_reinj_out,
+                 "n_wall_inward": o.n_wall_inward} for o in out]
 
     def run(self, steps):
         _check(self.L.potg_run(self.h, steps))

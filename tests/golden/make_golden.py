@@ -90,6 +90,20 @@ def cases():
     cs.append({"name": "sphere_dipoles", "bin": "pot_sphere_dip", "geometry": 0, "theta": 0.5,
                "B": [0.0, 0.0, 20.0], "args": dict(base), "steps": 60, "seed": 7,
                "dipoles": 1})
+    # the other integrators (1 velocity Verlet, 2 Euler-Richardson) and
+    # bouncing containment (pot.c:113-132)
+    cs.append({"name": "sphere_verlet_bz", "bin": "pot_sphere_verlet_bz", "geometry": 0,
+               "theta": 0.5, "B": [0.0, 0.0, 20.0], "args": dict(base), "steps": 60, "seed": 8,
+               "integrator": 1})
+    cs.append({"name": "sphere_er_bounce_bx", "bin": "pot_sphere_er_bounce_bx", "geometry": 0,
+               "theta": 0.5, "B": [30.0, 0.0, 0.0], "args": dict(base), "steps": 60, "seed": 9,
+               "integrator": 2, "bounce": 0.95})
+    cs.append({"name": "sphere_bounce", "bin": "pot_sphere_bounce", "geometry": 0, "theta": 0.5,
+               "B": [0.0, 0.0, 0.0], "args": dict(base), "steps": 60, "seed": 10,
+               "bounce": 0.95})
+    cs.append({"name": "prolate_verlet", "bin": "pot_prolate_verlet", "geometry": 1,
+               "theta": 0.5, "B": [0.0, 0.0, 0.0], "args": dict(pro), "steps": 60, "seed": 11,
+               "integrator": 1})
     return cs
 
 

@@ -28,7 +28,8 @@ def big_case(N, geometry=0, A=1.0, te=11600.0, ti=11600.0, B=(0.0, 0.0, 0.0), se
 def oracle_sim(case, inplace=0, rng_kind=O.RNG_WELL512, philox_seed=0):
     cfg = O.make_config(case["args"], geometry=case["geometry"], theta=case["theta"],
                         B=tuple(case["B"]), inplace=inplace, rng_kind=rng_kind,
-                        philox_seed=philox_seed, dipoles=case.get("dipoles", 0))
+                        philox_seed=philox_seed, dipoles=case.get("dipoles", 0),
+                        integrator=case.get("integrator", 0), bounce=case.get("bounce"))
     sim = O.Sim(cfg)
     sim.init_particles(splitmix64_words(case["seed"]))
     return sim
@@ -43,7 +44,8 @@ def gpu_config(case, seed=0):
                     Ti=L.or_strtold_to_double(a["-I"].encode()),
                     x_drift=float(a.get("-x", "0")), settle=0.0, geometry=case["geometry"],
                     A=float(a.get("-A", "1")), theta=case["theta"], B=tuple(case["B"]), seed=seed,
-                    dipoles=case.get("dipoles", 0))
+                    dipoles=case.get("dipoles", 0), integrator=case.get("integrator", 0),
+                    bounce=case.get("bounce"))
 
 
 def gpu_ctx(case, parts, precision=G.POTG_FP64, seed=0, q=0, t=0, chi_b=0.0):

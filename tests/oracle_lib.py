@@ -20,6 +20,7 @@ assert PARTICLE_DT.itemsize == 56
 DUST, ION, ELECTRON = 0, 1, 2
 SPHERE, PROLATE, OBLATE = 0, 1, 2
 RNG_WELL512, RNG_PHILOX = 0, 1
+BORIS, VERLET, EULER_RICHARDSON = 0, 1, 2
 
 
 class Config(C.Structure):
@@ -30,7 +31,8 @@ class Config(C.Structure):
                 ("theta", C.c_double), ("B", C.c_double * 3), ("max_lin_dev", C.c_double),
                 ("mime", C.c_double), ("grain_enabled", C.c_int32), ("sceptic", C.c_int32),
                 ("inplace", C.c_int32), ("rng_kind", C.c_int32), ("philox_seed", C.c_uint64),
-                ("record_macro", C.c_int32), ("dipoles", C.c_int32)]
+                ("record_macro", C.c_int32), ("dipoles", C.c_int32),
+                ("integrator", C.c_int32), ("out_bounce", C.c_int32), ("bounce_coe", C.c_double)]
 
 
 class WalkStats(C.Structure):
@@ -41,7 +43,7 @@ class WalkStats(C.Structure):
 class StepOut(C.Structure):
     _fields_ = [("abs_q", C.c_int32), ("abs_e", C.c_uint32), ("abs_i", C.c_uint32),
                 ("n_pushed", C.c_uint32), ("n_reinj_abs", C.c_uint32),
-                ("n_reinj_out", C.c_uint32), ("start", C.c_uint32), ("pad", C.c_uint32)]
+                ("n_reinj_out", C.c_uint32), ("start", C.c_uint32), ("n_wall_inward", C.c_uint32)]
 
 
 class Prng(C.Structure):
@@ -119,7 +121,7 @@ def dptr(a):
 
 def make_config(args, geometry=SPHERE, theta=0.5, B=(0.0, 0.0, 0.0), inplace=1,
                 rng_kind=0, philox_seed=0, record_macro=1, grain_enabled=1, sceptic=1,
-                dipoles=0):
+                dipoles=0, integrator=0, bounce=None):
     """Config from reference-style CLI values (strings), parsed as pot.c does."""
     L = lib()
     cfg = Config()
@@ -147,6 +149,9 @@ def make_config(args, geometry=SPHERE, theta=0.5, B=(0.0, 0.0, 0.0), inplace=1,
     cfg.philox_seed = philox_seed
     cfg.record_macro = record_macro
     cfg.dipoles = dipoles
+    cfg.integrator = integrator
+    cfg.out_bounce = 0 if bounce is None else 1
+    cfg.bounce_coe = 0.0 if bounce is None else bounce
     return cfg
 
 

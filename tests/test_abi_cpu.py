@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header():
-    assert ctypes.sizeof(G.Config) == 8 + 4 + 4 + 8 * 15 + 4 + 4 + 8 + 4 + 4
+    assert ctypes.sizeof(G.Config) == 8 + 4 + 4 + 8 * 15 + 4 + 4 + 8 + 4 * 4 + 8
     assert G.PARTICLE_DT.itemsize == 56
     assert ctypes.sizeof(G.Grain) == 4 + 4 + 8 + 24 + 8 + 24
     assert ctypes.sizeof(G.StepOut) == 32

@@ -54,7 +54,7 @@ def run_pot(tmp, args, name):
     return d
 
 
-def oracle_macro_lines(case, seed, steps):
+def oracle_macro_lines(case, seed, steps, integrator=0, bounce=None):
     sim = O.Sim(O.make_config(case["args"], geometry=0, theta=0.8, inplace=0))
     w = splitmix64_words(seed)
     sim.init_particles(w)
@@ -62,7 +62,8 @@ def oracle_macro_lines(case, seed, steps):
     sim.L.or_get_prng(sim.h, C.byref(pr))
     key = stream_key([pr.a, pr.b, pr.c, pr.d, pr.index] + list(pr.state))
     sim = O.Sim(O.make_config(case["args"], geometry=0, theta=0.8, inplace=0,
-                              rng_kind=O.RNG_PHILOX, philox_seed=key))
+                              rng_kind=O.RNG_PHILOX, philox_seed=key, integrator=integrator,
+                              bounce=bounce))
     sim.init_particles(w)
     L = macro_lib()
     lines = []
@@ -107,6 +108,28 @@ def test_pot_program_matches_oracle(tmp_path):
     assert "opening angle parameter: 0.8" in open(out / "pot-run-descrip.txt").read()
     hdr = open(tmp_path / "final.bin", "rb").readline().split()
     assert hdr[0] == b"C" and hdr[1] == b"1" and hdr[4] == str(steps).encode()
+
+
+@pytest.mark.parametrize("opts,integrator,bounce,line", [
+    (["--verlet"], 1, None, "numerical integrator: velocity Verlet"),
+    (["--euler-richardson", "--bounce", "0.95"], 2, 0.95, "containment:          bouncing (c.o.e. 0.95)"),
+])
+def test_pot_program_other_integrators(tmp_path, opts, integrator, bounce, line):
+    """The compile-time integrator and containment switches of the reference
+    (pot.c:113-132) as options: macro records against the oracle, and the
+    run description says which were used (pot.c:3751-3784)."""
+    case = big_case(2001, seed=6)
+    seed = 4343
+    (tmp_path / "seed.bin").write_bytes(struct.pack("<16I", *splitmix64_words(seed)))
+    steps = 8
+    out = run_pot(tmp_path, case_args(case) + ["-i", str(steps), "--fp64", "--seed-file",
+                                               str(tmp_path / "seed.bin")] + opts, "run")
+    got = open(out / "pot-macroscopic.dat").read().splitlines(True)
+    assert len(got) == steps + 1
+    want = oracle_macro_lines(case, seed, steps, integrator=integrator, bounce=bounce)
+    for t in range(steps):
+        close_lines(got[t + 1], want[t])
+    assert line in open(out / "pot-run-descrip.txt").read()
 
 
 def test_resumed_run_continues_the_straight_run(tmp_path):

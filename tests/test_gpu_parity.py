@@ -337,3 +337,67 @@ def test_upload_order_and_reupload_keep_instances_apart():
         else:
             assert np.array_equal(p["species"], states[k]["species"])
             assert not np.array_equal(p["x"], states[k]["x"])  # stepped once
+
+
+def test_bounce_off_the_wall_matches_oracle():
+    """OUT_BOUNCE (pot.c:2036-2069): an electron just outside the wall and
+    moving inwards is left where it is and reported (the reference warns,
+    pot.c:2724-2728); one moving outwards has its velocity reflected in
+    spherical components and goes back to its start position.  Both against
+    the oracle."""
+    case = next(c for c in CASES if c["name"] == "sphere_bounce")
+    sim = oracle_sim(case, inplace=0, rng_kind=O.RNG_PHILOX, philox_seed=5)
+    parts = sim.particles().copy()
+    R = float(case["args"]["-r"]) * float(case["args"]["-m"])
+    N = len(parts)
+    for j, (fac, sign) in ((N - 1, (1.02, -1.0)), (N - 2, (1.01, 1.0))):
+        assert parts["species"][j] == 2  # electrons: stepped every step
+        d = np.array([parts["x"][j], parts["y"][j], parts["z"][j]]) - R
+        u = d / np.linalg.norm(d)
+        speed = np.sqrt(parts["vx"][j] ** 2 + parts["vy"][j] ** 2 + parts["vz"][j] ** 2)
+        parts["x"][j], parts["y"][j], parts["z"][j] = R + fac * R * u
+        parts["vx"][j], parts["vy"][j], parts["vz"][j] = sign * speed * u
+    sim.set_particles(parts)
+    ctx = gpu_ctx(case, parts, seed=5)
+    oo = sim.run_step()
+    og = ctx.step()[0]
+    assert oo.n_wall_inward == 1 and og["n_wall_inward"] == 1
+    po = sim.particles()
+    pg, _ = ctx.download(0)
+    # the outward electron bounced back to its start, velocity reflected
+    assert po["x"][N - 2] == parts["x"][N - 2] and pg["x"][N - 2] == parts["x"][N - 2]
+    v_r = (po["vx"][N - 2] * (parts["x"][N - 2] - R) + po["vy"][N - 2] * (parts["y"][N - 2] - R) +
+           po["vz"][N - 2] * (parts["z"][N - 2] - R))
+    assert v_r < 0
+    vscale = np.abs(po["vx"]).max()
+    for f in ("x", "y", "z"):
+        assert np.abs(pg[f] - po[f]).max() <= 1e-12 * R, f
+    for f in ("vx", "vy", "vz"):
+        assert np.abs(pg[f] - po[f]).max() <= 1e-10 * vscale, f
+
+
+@pytest.mark.parametrize("name", ["sphere_verlet_bz", "sphere_er_bounce_bx", "prolate_verlet"])
+def test_two_field_integrators_over_steps(name):
+    """Velocity Verlet and Euler-Richardson (pot.c:2532-2639) over several
+    steps against the oracle with the GPU's semantics: each step takes a
+    second field at the intermediate state, with the cells of the step's
+    tree and the leaves moved to the intermediate positions."""
+    case = next(c for c in CASES if c["name"] == name)
+    sim, ctx, oo, og = _step_both(case, steps=5)
+    for a, b in zip(oo, og):
+        assert b["abs_q"] == a.abs_q and b["n_pushed"] == a.n_pushed
+        assert b["n_wall_inward"] == a.n_wall_inward
+    po = sim.particles()
+    pg, gr = ctx.download(0)
+    R = float(case["args"]["-r"]) * float(case["args"]["-m"])
+    p0 = oracle_sim(case).particles()
+    moved = np.sqrt((po["x"] - p0["x"]) ** 2 + (po["y"] - p0["y"]) ** 2 +
+                    (po["z"] - p0["z"]) ** 2) > 0.05 * R
+    vscale = np.abs(po["vx"]).max()
+    for f in ("x", "y", "z"):
+        d = np.abs(pg[f] - po[f])
+        assert d[~moved].max() <= 1e-11 * R, f
+    for f in ("vx", "vy", "vz"):
+        d = np.abs(pg[f] - po[f])
+        assert d[~moved].max() <= 1e-9 * vscale, f
+    assert gr["q"] == sim.state()["q"]

@@ -75,7 +75,8 @@ def test_macro_records_match_reference(case):
     lines = open(os.path.join(GOLD, case["name"], "macro.dat")).read().splitlines(True)
     assert lines[0] == L.pot_macro_header().decode()
     cfg = O.make_config(case["args"], geometry=case["geometry"], theta=case["theta"],
-                        B=tuple(case["B"]), dipoles=case.get("dipoles", 0))
+                        B=tuple(case["B"]), dipoles=case.get("dipoles", 0),
+                        integrator=case.get("integrator", 0), bounce=case.get("bounce"))
     sim = O.Sim(cfg)
     sim.init_particles(splitmix64_words(case["seed"]))
     steps = min(case["steps"], len(lines) - 1)

@@ -42,6 +42,15 @@ def test_state_roundtrip_is_byte_identical(case, which, tmp_path):
     src = os.path.join(GOLD, case["name"], which)
     parts = np.zeros(N, G.PARTICLE_DT)
     st = SimState()
+    raw = open(src, "rb").read()
+    if raw[raw.index(b"\n") + 1] in b" \t\n\v\f\r":
+        # the reference's reader ends its header format with "\n"
+        # (pot.c:3923), which skips whitespace, so a binary PRNG record that
+        # starts with a whitespace byte is misread and the particle read
+        # comes up short: the reference cannot resume from such a file, and
+        # neither can this reader
+        assert L.pot_read_state(src.encode(), N, parts.ctypes.data, C.byref(st)) != 0
+        return
     assert L.pot_read_state(src.encode(), N, parts.ctypes.data, C.byref(st)) == 0
     hdr, prng, ref_parts = O.read_state_file(src, N)
     for f in ("x", "y", "z", "vx", "vy", "vz", "species"):

@@ -24,7 +24,8 @@ CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 
 def run_oracle(case, steps):
     cfg = O.make_config(case["args"], geometry=case["geometry"], theta=case["theta"],
-                        B=tuple(case["B"]), dipoles=case.get("dipoles", 0))
+                        B=tuple(case["B"]), dipoles=case.get("dipoles", 0),
+                        integrator=case.get("integrator", 0), bounce=case.get("bounce"))
     sim = O.Sim(cfg)
     sim.init_particles(splitmix64_words(case["seed"]))
     for _ in range(steps):
