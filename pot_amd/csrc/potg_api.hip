@@ -326,10 +326,18 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
     b.key_bits = 64;
-    // defaults until potg_tune or potg_set_tuning: 64 targets per
-    // wavefront, 8 blocks of 4 waves per CU
-    b.walk_blocks = 8 * device_cus();
-    b.group = 64;
+    // defaults until potg_tune or potg_set_tuning, by size: with enough
+    // 64-target tiles to give every SIMD several wavefronts (about 0.5M
+    // particles and up, half of them stepped each step), 64 targets per
+    // wavefront and 8 blocks of 4 waves per CU; below that the tiles would
+    // leave most of the chip idle, so 16 targets per wavefront and 4
+    // blocks per CU
+    {
+        const uint64_t tiles = n_total / 128u;
+        const bool large = tiles >= 16u * (uint64_t)device_cus();
+        b.walk_blocks = (large ? 8u : 4u) * (uint32_t)device_cus();
+        b.group = (large || precision == POTG_FP64) ? 64u : 16u;  // float64: always 64
+    }
 
     c->hp.resize(n_inst);
     uint32_t off = 0, tb = 0;

@@ -59,6 +59,7 @@ typedef struct pot_run {
     int integrator;         /* POTG_BORIS / POTG_VERLET / POTG_EULER_RICHARDSON */
     int out_bounce;         /* OUT_BOUNCE */
     double bounce_coe;      /* its value, the coefficient of restitution */
+    int tune;               /* time the walk's launch shapes (potg_tune) */
     int precision;          /* POTG_FP64 / POTG_MIXED */
     const char* seed_path;  /* RANDOMNESS_SOURCE */
     const char* out_dir;    /* OUTPUT_DIR */

@@ -24,6 +24,10 @@
  *   --euler-richardson  EULER_RICHARDSON (pot.c:122-128)
  *   --bounce COE     OUT_BOUNCE with coefficient of restitution COE:
  *                    particles bounce off the wall (pot.c:113-120)
+ *   --tune           time the walk's launch shapes after the first steps
+ *                    and keep the fastest (potg_tune); without it the
+ *                    shape follows the particle count, so runs repeat
+ *                    exactly
  *   --oblate         OBLATE instead of PROLATE (pot-spheroid only)
  *   --fp64           float64 walk (default: float32 walk with compensated
  *                    sums, float64 push)
@@ -67,6 +71,7 @@ static void set_defaults(pot_run* ru)
     ru->integrator = POTG_BORIS;
     ru->out_bounce = 0;
     ru->bounce_coe = 0.0;
+    ru->tune = 0;
     ru->precision = POTG_MIXED;
     ru->seed_path = "/dev/urandom";
     ru->out_dir = "./output/";
@@ -113,7 +118,7 @@ static void usage(const char* prog)
           stderr);
     if (SPHEROID_PROGRAM) fputs("  -A [NUMBER]   aspect ratio of the spheroid\n", stderr);
     fputs("  --theta X  --bx X --by X --bz X  --mime X  --no-grain  --dipoles  --fp64\n"
-          "  --verlet  --euler-richardson  --bounce COE\n"
+          "  --verlet  --euler-richardson  --bounce COE  --tune\n"
           "  --seed-file PATH  --output-dir DIR", stderr);
     fputs(SPHEROID_PROGRAM ? "  --oblate\n" : "\n", stderr);
 }
@@ -144,6 +149,10 @@ static int parse_args(int argc, char** argv, pot_run* ru)
             }
             if (!strcmp(o, "--fp64")) {
                 ru->precision = POTG_FP64;
+                continue;
+            }
+            if (!strcmp(o, "--tune")) {
+                ru->tune = 1;
                 continue;
             }
             if (!strcmp(o, "--verlet")) {
@@ -518,6 +527,16 @@ static int run(host_run* h, uint64_t t_completed)
             fprintf(stderr, "0:%lu: %s\n", t, potg_last_error());
             failed = 1;
             break;
+        }
+        /* once the first steps have warmed the caches: the walk's launch
+           shape that is fastest for this run */
+        if (ru->tune && t == 3) {
+            potg_tuning tt;
+            if (potg_tune(h->ctx, &tt) != POTG_OK) {
+                fprintf(stderr, "0:%lu: %s\n", t, potg_last_error());
+                failed = 1;
+                break;
+            }
         }
         /* the reference warns of each such particle (pot.c:2724-2728) */
         if (ru->out_bounce && so.n_wall_inward)
