@@ -210,9 +210,10 @@ int potg_direct_sum(potg_ctx* ctx, uint32_t inst, const uint32_t* idx, size_t n,
                     double* E);
 
 /* Time the field walk of the coming step under candidate launch parameters
-   (pair threshold, grid size), keep the fastest and report it in *out.  The
-   step itself is not taken: the state, step counter and work counters are
-   left as they were.  No counterpart in the reference. */
+   (targets per wavefront, grid size), keep the fastest and report it in
+   *out (out may be NULL).  The step itself is not taken: the state, step
+   counter and work counters are left as they were.  No counterpart in the
+   reference. */
 int potg_tune(potg_ctx* ctx, potg_tuning* out);
 /* Set or read the walk's launch parameters (set NULL to only read). */
 int potg_set_tuning(potg_ctx* ctx, const potg_tuning* set, potg_tuning* get);
