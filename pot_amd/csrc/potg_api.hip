@@ -378,8 +378,6 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.lvl_hist, (size_t)MAXD * LVL_BLOCKS);
     rc |= alloc_n(c, &b.lvl_off, (size_t)MAXD * LVL_BLOCKS);
     rc |= alloc_n(c, &b.lvl_start, MAXD + 1) | alloc_n(c, &b.by_level, ci);
-    rc |= alloc_n(c, &b.mcx, ci) | alloc_n(c, &b.mcy, ci) | alloc_n(c, &b.mcz, ci);
-    rc |= alloc_n(c, &b.mq, ci) | alloc_n(c, &b.maq, ci);
     b.n_rec = (uint32_t)(ci + nt + n_inst + 8);
     // the group walk packs a child range as first | (cnt - 1) << 29
     if (!rc && precision == POTG_MIXED_GROUP && b.n_rec >= (1u << 29)) {
@@ -394,7 +392,6 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     bool any_dip = false;
     for (uint32_t i = 0; i < n_inst; i++) any_dip |= configs[i].dipoles != 0;
     if (any_dip) {
-        rc |= alloc_n(c, &b.mpx, ci) | alloc_n(c, &b.mpy, ci) | alloc_n(c, &b.mpz, ci);
         rc |= alloc_n(c, &b.dip64, 3 * (size_t)b.n_rec);
         if (precision != POTG_FP64) rc |= alloc_n(c, &b.dip32, b.n_rec);
     }
@@ -708,8 +705,12 @@ static int run_points(potg_ctx* c, uint32_t inst, const double* xyz, const uint3
     Bufs& b = c->b;
     std::vector<uint32_t> vi, vs;
     std::vector<uint32_t> inv(b.n_total);
-    if (kind == 0 && exclude)
-        HIPCHK(hipMemcpy(inv.data(), b.inv, inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (kind == 0 && exclude) {
+        launch_build_inverse(b, c->stream);
+        HIPCHK(hipMemcpyAsync(inv.data(), b.inv, inv.size() * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     const uint32_t off = c->hp[inst].offset;
     for (size_t s0 = 0; s0 < n; s0 += b.q_cap) {
         const uint32_t m = (uint32_t)((n - s0) < b.q_cap ? (n - s0) : b.q_cap);
@@ -797,7 +798,10 @@ int potg_direct_sum(potg_ctx* c, uint32_t inst, const uint32_t* idx, size_t n, d
         return set_err("potg_direct_sum: bad arguments"), POTG_ERR_ARG;
     Bufs& b = c->b;
     std::vector<uint32_t> inv(b.n_total), vs;
-    HIPCHK(hipMemcpy(inv.data(), b.inv, inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    launch_build_inverse(b, c->stream);
+    HIPCHK(hipMemcpyAsync(inv.data(), b.inv, inv.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     const uint32_t off = c->hp[inst].offset;
     for (size_t s0 = 0; s0 < n; s0 += b.q_cap) {
         const uint32_t m = (uint32_t)((n - s0) < b.q_cap ? (n - s0) : b.q_cap);

@@ -84,6 +84,7 @@ struct InstState {
     double epe[3];
     uint32_t start;      // first stepped reference index of the current step
     uint32_t root;       // walk id of the tree root
+    uint32_t grain_slot; // sorted slot of the grain (index 0) after the step's sort
     // per-step accumulators (cleared by the step's first kernel)
     int32_t acc_q;
     uint32_t acc_e, acc_i, n_pushed, n_reinj_abs, n_reinj_out, n_wall_inward;

@@ -7,15 +7,6 @@
 
 namespace potg {
 
-// Per-internal-node moment arrays of the build (and their record copies).
-struct NodeMoments {
-    double *cx, *cy, *cz, *px, *py, *pz;
-    int32_t* q;
-    uint32_t* aq;
-    double* dip64;
-    float4* dip32;
-};
-
 // Particle state in sorted (octree) order, structure of arrays.
 struct PState {
     double *x, *y, *z, *vx, *vy, *vz;
@@ -59,10 +50,6 @@ struct Bufs {
     uint32_t* lvl_start;           // first slot of each depth in by_level
     uint32_t* by_level;  // internal node ids grouped by depth
     double* size2;
-    double *mcx, *mcy, *mcz;
-    int32_t* mq;
-    uint32_t* maq;
-    double *mpx, *mpy, *mpz;  // dipole moments of internal nodes (DIPOLES only)
     NodeRec64* rec64;
     NodeRec32* rec32;
     double* dip64;     // dipole of each record, 3 per record (DIPOLES only)

@@ -139,64 +139,164 @@ __global__ __launch_bounds__(BT) void k_scan_down(const uint32_t* __restrict__ i
 }
 
 // ---- radix sort -------------------------------------------------------
+//
+// 8-bit digits, three launches per pass: the digit counts of every
+// 4096-element tile (k_rs_count), a scan of each digit's counts over the
+// tiles (k_rs_scan, one block per digit), and k_rs_scatter, which ranks its
+// tile stably by digit with wave ballots, stages it in LDS in digit order
+// and then stores every digit's run to its global place, so that
+// neighbouring lanes store to neighbouring addresses.
 
-__global__ __launch_bounds__(BT) void k_radix_hist(const uint64_t* __restrict__ keys, uint32_t n,
-                                                    int shift, uint32_t nb,
-                                                    uint32_t* __restrict__ ghist)
+constexpr int RS_IPT = 16;                     // elements per thread
+constexpr int RS_TILE = BT * RS_IPT;           // elements per block
+constexpr int RS_WAVES = BT / WAVE;
+constexpr int RS_WTILE = RS_TILE / RS_WAVES;   // contiguous elements per wavefront
+
+inline uint32_t rs_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+__global__ __launch_bounds__(BT) void k_rs_count(const uint64_t* __restrict__ keys, uint32_t n,
+                                                  int shift, uint32_t nb,
+                                                  uint32_t* __restrict__ cnt)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * TILE;
+    __shared__ uint32_t h[RS_WAVES][256];
 #pragma unroll
-    for (int k = 0; k < IPT; k++) {
-        uint32_t i = base + k * BT + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
-    }
+    for (int k = 0; k < RS_WAVES; k++) h[k][threadIdx.x] = 0;
     __syncthreads();
-    ghist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    const int w = threadIdx.x / WAVE;
+    const uint32_t base = blockIdx.x * RS_TILE;
+    uint32_t d[RS_IPT];
+#pragma unroll
+    for (int k = 0; k < RS_IPT; k++) {
+        const uint32_t i = base + k * BT + threadIdx.x;
+        d[k] = i < n ? (uint32_t)(keys[i] >> shift) & 255u : 256u;
+    }
+#pragma unroll
+    for (int k = 0; k < RS_IPT; k++)
+        if (d[k] < 256u) atomicAdd(&h[w][d[k]], 1u);
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) c += h[k][threadIdx.x];
+    cnt[threadIdx.x * nb + blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(BT) void k_radix_scatter(const uint64_t* __restrict__ kin,
-                                                       const uint32_t* __restrict__ vin,
-                                                       uint64_t* __restrict__ kout,
-                                                       uint32_t* __restrict__ vout, uint32_t n,
-                                                       int shift, uint32_t nb,
-                                                       const uint32_t* __restrict__ goff)
+// in place: cnt[d][b] becomes the count of digit d in the tiles before b;
+// tot[d] = the count of digit d over all tiles
+__global__ __launch_bounds__(BT) void k_rs_scan(uint32_t* __restrict__ cnt, uint32_t nb,
+                                                 uint32_t* __restrict__ tot)
 {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[BT / WAVE][256];
+    __shared__ uint32_t sh_w[BT / WAVE];
+    uint32_t* c = cnt + (size_t)blockIdx.x * nb;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += BT * 4) {
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = b0 + threadIdx.x * 4 + k;
+            v[k] = i < nb ? c[i] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t off = carry + block_excl_scan(sum, sh_w, &total);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = b0 + threadIdx.x * 4 + k;
+            if (i < nb) c[i] = off;
+            off += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(BT) void k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                    const uint32_t* __restrict__ vin,
+                                                    uint64_t* __restrict__ kout,
+                                                    uint32_t* __restrict__ vout, uint32_t n,
+                                                    int shift, uint32_t nb,
+                                                    const uint32_t* __restrict__ cnt,
+                                                    const uint32_t* __restrict__ tot)
+{
+    __shared__ uint64_t skey[RS_TILE];
+    __shared__ uint32_t sval[RS_TILE];
+    __shared__ uint32_t wc[RS_WAVES][256];  // running digit counts per wavefront
+    __shared__ uint32_t gdelta[256];        // global index - LDS index, per digit
+    __shared__ uint32_t sh_w[BT / WAVE];
+
     const int w = threadIdx.x / WAVE;
-    run[threadIdx.x] = goff[threadIdx.x * nb + blockIdx.x];
-    const uint32_t base = blockIdx.x * TILE;
-    for (int r = 0; r < IPT; r++) {
-        for (int k = 0; k < BT / WAVE; k++) wcnt[k][threadIdx.x] = 0;
-        __syncthreads();
-        const uint32_t i = base + r * BT + threadIdx.x;
-        const bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0;
-        uint32_t val = valid ? vin[i] : 0;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) wc[k][threadIdx.x] = 0;
+    // where this tile's run of digit threadIdx.x starts in the output (the
+    // scan's barriers also publish the zeroed counts)
+    const uint32_t gstart =
+        block_excl_scan(tot[threadIdx.x], sh_w, nullptr) + cnt[threadIdx.x * nb + blockIdx.x];
+
+    // wavefront w ranks the elements [base, base + RS_WTILE) in rounds of 64
+    const uint32_t base = blockIdx.x * RS_TILE + w * RS_WTILE;
+    uint64_t key[RS_IPT];
+    uint32_t val[RS_IPT], rnk[RS_IPT];
+#pragma unroll
+    for (int r = 0; r < RS_IPT; r++) {
+        const uint32_t i = base + r * WAVE + lane;
+        key[r] = i < n ? kin[i] : 0ull;
+        val[r] = i < n ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_IPT; r++) {
+        const bool valid = base + r * WAVE + lane < n;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t m = ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            uint64_t bb = ballot((d >> b) & 1u);
+            const uint64_t bb = ballot((d >> b) & 1u);
             m &= ((d >> b) & 1u) ? bb : ~bb;
         }
-        const uint32_t rank = __popcll(m & lanemask_lt());
-        if (valid && rank == 0) wcnt[w][d] = __popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t pre = run[d];
-            for (int k = 0; k < w; k++) pre += wcnt[k][d];
-            kout[pre + rank] = key;
-            vout[pre + rank] = val;
+        const uint32_t below = __popcll(m & lanemask_lt());
+        // every lane reads the count before the lowest lane of its digit
+        // adds the round's run to it (one wavefront: LDS accesses in order)
+        const uint32_t run = wc[w][d];
+        rnk[r] = run + below;
+        if (valid && below == 0) wc[w][d] = run + __popcll(m);
+    }
+    __syncthreads();
+    {
+        // per digit: start of each wavefront's run in the tile, and of the
+        // tile's run in LDS (digit order)
+        const uint32_t d = threadIdx.x;
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) {
+            const uint32_t c = wc[k][d];
+            wc[k][d] = sum;
+            sum += c;
         }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int k = 0; k < BT / WAVE; k++) add += wcnt[k][threadIdx.x];
-        run[threadIdx.x] += add;
-        __syncthreads();
+        const uint32_t lstart = block_excl_scan(sum, sh_w, nullptr);
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) wc[k][d] += lstart;
+        gdelta[d] = gstart - lstart;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_IPT; r++) {
+        if (base + r * WAVE + lane < n) {
+            const uint32_t p = wc[w][(uint32_t)(key[r] >> shift) & 255u] + rnk[r];
+            skey[p] = key[r];
+            sval[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t t0 = blockIdx.x * RS_TILE;
+    const uint32_t tile_n = n - t0 < (uint32_t)RS_TILE ? n - t0 : (uint32_t)RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_IPT; r++) {
+        const uint32_t p = r * BT + threadIdx.x;
+        if (p < tile_n) {
+            const uint64_t k = skey[p];
+            const uint32_t o = p + gdelta[(uint32_t)(k >> shift) & 255u];
+            kout[o] = k;
+            vout[o] = sval[p];
+        }
     }
 }
 
@@ -230,18 +330,19 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_
     k_scan_down<<<nb, BT, 0, s>>>(in, n, n_dev, temp, out, add);
 }
 
-size_t radix_temp_words(uint32_t n) { return 256ull * (nblocks(n) + 1) * 2 + 64; }
+size_t radix_temp_words(uint32_t n) { return 256ull * (rs_blocks(n) + 1) + 64; }
 
 void radix_sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt,
                       uint32_t n, int end_bit, uint32_t* temp, hipStream_t s)
 {
-    const uint32_t nb = nblocks(n) ? nblocks(n) : 1;
-    uint32_t* hist = temp;
-    uint32_t* scan_tmp = temp + 256ull * nb;
+    if (n == 0) return;
+    const uint32_t nb = rs_blocks(n);
+    uint32_t* cnt = temp;
+    uint32_t* tot = temp + 256ull * nb;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        k_radix_hist<<<nb, BT, 0, s>>>(keys, n, shift, nb, hist);
-        exclusive_scan(hist, hist, 256u * nb, nullptr, nullptr, nullptr, scan_tmp, 0, s);
-        k_radix_scatter<<<nb, BT, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, nb, hist);
+        k_rs_count<<<nb, BT, 0, s>>>(keys, n, shift, nb, cnt);
+        k_rs_scan<<<256, BT, 0, s>>>(cnt, nb, tot);
+        k_rs_scatter<<<nb, BT, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, nb, cnt, tot);
         uint64_t* tk = keys; keys = keys_alt; keys_alt = tk;
         uint32_t* tv = vals; vals = vals_alt; vals_alt = tv;
     }

@@ -575,7 +575,7 @@ __global__ void k_target_flags(const InstParams* __restrict__ P, const InstState
 
 __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals* g, uint32_t n_inst,
                          int compute_epe, const double* __restrict__ epe_part,
-                         const double* __restrict__ phi, const uint32_t* __restrict__ inv)
+                         const double* __restrict__ phi)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n_inst) {
@@ -602,7 +602,7 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
             // the grain's own potential energy with its updated charge
             // (pot.c:2852-2863)
             if (compute_epe)
-                s.epe[0] = s.grain_q ? ((double)s.grain_q * QE) * phi[inv[p.offset]] : 0.0;
+                s.epe[0] = s.grain_q ? ((double)s.grain_q * QE) * phi[s.grain_slot] : 0.0;
         }
         s.out_q = s.acc_q;
         s.out_e = s.acc_e;
@@ -790,8 +790,7 @@ void launch_half(Bufs& b, hipStream_t s)
 
 void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s)
 {
-    k_finish<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst, compute_epe, b.epe_part, b.phi,
-                                            b.inv);
+    k_finish<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst, compute_epe, b.epe_part, b.phi);
 }
 
 void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n, hipStream_t s)
@@ -804,6 +803,7 @@ void launch_scatter_upload(Bufs& b, uint32_t inst, const double* aos, uint32_t n
 
 void launch_gather_download(Bufs& b, uint32_t inst, double* aos, uint32_t n, hipStream_t s)
 {
+    launch_build_inverse(b, s);
     uint32_t offset = 0;
     (void)hipMemcpyAsync(&offset, &b.P[inst].offset, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);

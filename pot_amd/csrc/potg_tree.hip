@@ -155,8 +155,10 @@ __global__ void k_tiefix(const uint64_t* __restrict__ key, uint32_t* __restrict_
     }
 }
 
+// the particles into sorted order; the grain of each instance notes its
+// slot (the potential-energy pass reads the grain's potential there)
 __global__ void k_permute(const uint32_t* __restrict__ perm, PState src, PState dst, uint32_t n,
-                          uint32_t* __restrict__ inv)
+                          const InstParams* __restrict__ P, InstState* __restrict__ S)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -168,10 +170,12 @@ __global__ void k_permute(const uint32_t* __restrict__ perm, PState src, PState 
     dst.vy[i] = src.vy[j];
     dst.vz[i] = src.vz[j];
     const uint32_t r = src.ref[j];
+    const uint16_t inst = src.inst[j];
+    const uint8_t spec = src.spec[j];
     dst.ref[i] = r;
-    dst.inst[i] = src.inst[j];
-    dst.spec[i] = src.spec[j];
-    inv[r] = i;
+    dst.inst[i] = inst;
+    dst.spec[i] = spec;
+    if (spec == 0 && r == P[inst].offset) S[inst].grain_slot = i;
 }
 
 __global__ void k_lcp(const uint64_t* __restrict__ key, PState st, const InstParams* __restrict__ P,
@@ -553,80 +557,53 @@ __global__ __launch_bounds__(BT) void k_level_scatter(const int32_t* __restrict_
 // the moments of internal node k, its children folded in the reference
 // order (compute_node_values / update_node_moments, pot.c:1290-1402):
 // charge, |charge|, centre of |charge| and, for DIPOLES instances, the
-// dipole moment about that centre
+// dipole moment about that centre.  The children are read from their
+// records, contiguous in that order and finished before this level: the
+// leaves by k_leaf_records, internal children by the deeper levels.
 __device__ void level_node_moments(uint32_t k, PState st, const InstParams* __restrict__ P,
-                                   const InstState* __restrict__ S, uint32_t cap,
-                                   const int32_t* __restrict__ level,
-                                   const uint32_t* __restrict__ owner,
-                                   const uint32_t* __restrict__ slots,
+                                   uint32_t cap, const uint32_t* __restrict__ owner,
                                    const uint32_t* __restrict__ ccount,
                                    const uint32_t* __restrict__ cfirst,
                                    const uint32_t* __restrict__ wid_int,
-                                   const double* __restrict__ size2, NodeMoments m,
+                                   const double* __restrict__ size2,
                                    NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
+                                   double* __restrict__ dip64, float4* __restrict__ dip32,
                                    uint32_t n_rec)
 {
     if (k >= cap) return;
+    const uint32_t w = wid_int[k];
+    if (w >= n_rec) return;
+    const uint32_t first = cfirst[k];
+    const uint32_t end = first + ccount[k] < n_rec ? first + ccount[k] : n_rec;
     const uint32_t inst = st.inst[owner[k]];
-    const int32_t gq = S[inst].grain_q;
-    const bool dip = P[inst].dipoles && m.px;
+    const bool dip = dip64 && P[inst].dipoles;
     double cx = 0.0, cy = 0.0, cz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     int q = 0;
     uint32_t aq = 0;
-    for (int s = 0; s < 8; s++) {
-        const uint32_t c = slots[8 * k + s];
-        if (c == NONE) continue;
-        int qc;
-        uint32_t aqc;
-        double ccx, ccy, ccz, cpx = 0.0, cpy = 0.0, cpz = 0.0;
-        if (c & LEAF_BIT) {
-            const uint32_t j = c & ~LEAF_BIT;
-            qc = charge_of(st.spec[j], gq);
-            aqc = (uint32_t)abs(qc);
-            ccx = st.x[j];
-            ccy = st.y[j];
-            ccz = st.z[j];
-        } else {
-            qc = m.q[c];
-            aqc = m.aq[c];
-            ccx = m.cx[c];
-            ccy = m.cy[c];
-            ccz = m.cz[c];
-            if (dip) {
-                cpx = m.px[c];
-                cpy = m.py[c];
-                cpz = m.pz[c];
-            }
-        }
+    for (uint32_t j = first; j < end; j++) {
+        const NodeRec64 c = r64[j];
+        const int qc = c.q;
+        const uint32_t aqc = c.cnt == LEAF ? (uint32_t)abs(qc) : c.part;
         if (aqc) {
             // update_node_moments (pot.c:1290-1310), same float64 operations
             const double ox = cx, oy = cy, oz = cz;
             const double den = (double)(aq + aqc);
-            cx = ((double)aq * cx + (double)aqc * ccx) / den;
-            cy = ((double)aq * cy + (double)aqc * ccy) / den;
-            cz = ((double)aq * cz + (double)aqc * ccz) / den;
+            cx = ((double)aq * cx + (double)aqc * c.cx) / den;
+            cy = ((double)aq * cy + (double)aqc * c.cy) / den;
+            cz = ((double)aq * cz + (double)aqc * c.cz) / den;
             if (dip) {
                 // shift the dipole to the new centre, then add the child's
+                // (zero for a leaf)
                 px -= (cx - ox) * (double)q;
                 py -= (cy - oy) * (double)q;
                 pz -= (cz - oz) * (double)q;
-                px += cpx - (cx - ccx) * (double)qc;
-                py += cpy - (cy - ccy) * (double)qc;
-                pz += cpz - (cz - ccz) * (double)qc;
+                px += dip64[3 * (size_t)j] - (cx - c.cx) * (double)qc;
+                py += dip64[3 * (size_t)j + 1] - (cy - c.cy) * (double)qc;
+                pz += dip64[3 * (size_t)j + 2] - (cz - c.cz) * (double)qc;
             }
             q += qc;
             aq += aqc;
         }
-    }
-    m.cx[k] = cx;
-    m.cy[k] = cy;
-    m.cz[k] = cz;
-    m.q[k] = q;
-    m.aq[k] = aq;
-    if (dip) {
-        m.px[k] = px;
-        m.py[k] = py;
-        m.pz[k] = pz;
     }
     NodeRec64 e;
     e.cx = cx;
@@ -634,57 +611,53 @@ __device__ void level_node_moments(uint32_t k, PState st, const InstParams* __re
     e.cz = cz;
     e.size2 = size2[k];
     e.q = q;
-    e.first = cfirst[k];
+    e.first = first;
     e.cnt = ccount[k];
     e.part = aq;
-    const uint32_t w = wid_int[k];
-    if (w >= n_rec) return;
     r64[w] = e;
     if (r32) put32(r32, w, P[inst], cx, cy, cz, e.size2, q, e.first, e.cnt, aq);
-    if (m.dip64) {
-        m.dip64[3 * (size_t)w] = px;
-        m.dip64[3 * (size_t)w + 1] = py;
-        m.dip64[3 * (size_t)w + 2] = pz;
+    if (dip64) {
+        dip64[3 * (size_t)w] = px;
+        dip64[3 * (size_t)w + 1] = py;
+        dip64[3 * (size_t)w + 2] = pz;
     }
     // the float32 walk works in units of R: p / R
-    if (m.dip32)
-        m.dip32[w] = make_float4((float)(px * P[inst].invR), (float)(py * P[inst].invR),
-                                 (float)(pz * P[inst].invR), 0.f);
+    if (dip32)
+        dip32[w] = make_float4((float)(px * P[inst].invR), (float)(py * P[inst].invR),
+                               (float)(pz * P[inst].invR), 0.f);
 }
 
 // the moments of the internal nodes at depth L; the deeper levels were
 // finished by earlier launches
 __global__ void k_level_moments(int L, PState st, const InstParams* __restrict__ P,
-                                const InstState* __restrict__ S, const Globals* __restrict__ g,
-                                uint32_t cap, const int32_t* __restrict__ level,
+                                const Globals* __restrict__ g, uint32_t cap,
                                 const uint32_t* __restrict__ owner,
-                                const uint32_t* __restrict__ slots,
                                 const uint32_t* __restrict__ ccount,
                                 const uint32_t* __restrict__ cfirst,
                                 const uint32_t* __restrict__ wid_int,
-                                const double* __restrict__ size2, NodeMoments m,
-                                NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
-                                uint32_t n_rec, const uint32_t* __restrict__ lstart,
+                                const double* __restrict__ size2, NodeRec64* __restrict__ r64,
+                                NodeRec32* __restrict__ r32, double* __restrict__ dip64,
+                                float4* __restrict__ dip32, uint32_t n_rec,
+                                const uint32_t* __restrict__ lstart,
                                 const uint32_t* __restrict__ by_level)
 {
     if ((uint32_t)L + 1 > g->max_depth) return;  // no internal node this deep
     const uint32_t lo = lstart[L], hi = lstart[L + 1];
     for (uint32_t idx = lo + blockIdx.x * blockDim.x + threadIdx.x; idx < hi;
          idx += gridDim.x * blockDim.x)
-        level_node_moments(by_level[idx], st, P, S, cap, level, owner, slots, ccount, cfirst,
-                           wid_int, size2, m, r64, r32, n_rec);
+        level_node_moments(by_level[idx], st, P, cap, owner, ccount, cfirst, wid_int, size2, r64,
+                           r32, dip64, dip32, n_rec);
 }
 
 // levels Lhi down to Llo by one block, a barrier between levels: for the
 // thinly populated deep and shallow levels, where a launch per level would
 // cost more than the work
 __global__ __launch_bounds__(1024) void k_level_moments_block(
-    int Lhi, int Llo, PState st, const InstParams* __restrict__ P, const InstState* __restrict__ S,
-    const Globals* __restrict__ g, uint32_t cap, const int32_t* __restrict__ level,
-    const uint32_t* __restrict__ owner, const uint32_t* __restrict__ slots,
-    const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ cfirst,
-    const uint32_t* __restrict__ wid_int, const double* __restrict__ size2, NodeMoments m,
-    NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32, uint32_t n_rec,
+    int Lhi, int Llo, PState st, const InstParams* __restrict__ P, const Globals* __restrict__ g,
+    uint32_t cap, const uint32_t* __restrict__ owner, const uint32_t* __restrict__ ccount,
+    const uint32_t* __restrict__ cfirst, const uint32_t* __restrict__ wid_int,
+    const double* __restrict__ size2, NodeRec64* __restrict__ r64, NodeRec32* __restrict__ r32,
+    double* __restrict__ dip64, float4* __restrict__ dip32, uint32_t n_rec,
     const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ by_level)
 {
     const uint32_t depth = g->max_depth;
@@ -692,8 +665,8 @@ __global__ __launch_bounds__(1024) void k_level_moments_block(
         if ((uint32_t)L + 1 > depth) continue;  // uniform over the block
         const uint32_t lo = lstart[L], hi = lstart[L + 1];
         for (uint32_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x)
-            level_node_moments(by_level[idx], st, P, S, cap, level, owner, slots, ccount, cfirst,
-                               wid_int, size2, m, r64, r32, n_rec);
+            level_node_moments(by_level[idx], st, P, cap, owner, ccount, cfirst, wid_int, size2,
+                               r64, r32, dip64, dip32, n_rec);
         // the next level up reads what this one wrote
         __syncthreads();
     }
@@ -724,7 +697,7 @@ void launch_keys_sort_permute(Bufs& b, hipStream_t s)
     k_keys<<<grid(b.n_total), BT, 0, s>>>(b.P, b.st, b.n_total, b.key_levels, b.key, b.perm, b.g);
     radix_sort_pairs(b.key, b.key_alt, b.perm, b.perm_alt, b.n_total, b.key_bits, b.sort_tmp, s);
     k_tiefix<<<grid(b.n_tree), BT, 0, s>>>(b.key, b.perm, b.P, b.st, b.n_tree, b.g);
-    k_permute<<<grid(b.n_total), BT, 0, s>>>(b.perm, b.st, b.alt, b.n_total, b.inv);
+    k_permute<<<grid(b.n_total), BT, 0, s>>>(b.perm, b.st, b.alt, b.n_total, b.P, b.S);
     PState t = b.st;
     b.st = b.alt;
     b.alt = t;
@@ -755,7 +728,6 @@ void launch_tree(Bufs& b, hipStream_t s)
         float4* d32 = b.precision ? b.dip32 : nullptr;
         k_leaf_records<<<grid(n), BT, 0, s>>>(b.st, b.P, b.S, n, b.n_rec, b.wid_leaf, b.rec64,
                                               r32, b.dip64, d32);
-        NodeMoments m{b.mcx, b.mcy, b.mcz, b.mpx, b.mpy, b.mpz, b.mq, b.maq, b.dip64, d32};
         const uint32_t cap = b.cap_int;
         const uint32_t gb = grid(cap) < 2048u ? grid(cap) : 2048u;
         k_level_count<<<LVL_BLOCKS, BT, 0, s>>>(b.level, b.g, cap, b.lvl_hist);
@@ -765,18 +737,18 @@ void launch_tree(Bufs& b, hipStream_t s)
         // any plasma (their cells are a few mean spacings wide or less), as
         // do those above LVL_WIDE_HI: one block each; the levels between
         // get a launch of their own
-        k_level_moments_block<<<1, 1024, 0, s>>>(MAXD - 1, LVL_WIDE_LO, b.st, b.P, b.S, b.g, cap,
-                                                 b.level, b.owner, b.slots, b.ccount, b.cfirst,
-                                                 b.wid_int, b.size2, m, b.rec64, r32, b.n_rec,
-                                                 b.lvl_start, b.by_level);
+        k_level_moments_block<<<1, 1024, 0, s>>>(MAXD - 1, LVL_WIDE_LO, b.st, b.P, b.g, cap,
+                                                 b.owner, b.ccount, b.cfirst, b.wid_int, b.size2,
+                                                 b.rec64, r32, b.dip64, d32, b.n_rec, b.lvl_start,
+                                                 b.by_level);
         for (int L = LVL_WIDE_LO - 1; L > LVL_WIDE_HI; L--)
-            k_level_moments<<<gb, BT, 0, s>>>(L, b.st, b.P, b.S, b.g, cap, b.level, b.owner,
-                                              b.slots, b.ccount, b.cfirst, b.wid_int, b.size2, m,
-                                              b.rec64, r32, b.n_rec, b.lvl_start, b.by_level);
-        k_level_moments_block<<<1, 1024, 0, s>>>(LVL_WIDE_HI, 0, b.st, b.P, b.S, b.g, cap,
-                                                 b.level, b.owner, b.slots, b.ccount, b.cfirst,
-                                                 b.wid_int, b.size2, m, b.rec64, r32, b.n_rec,
-                                                 b.lvl_start, b.by_level);
+            k_level_moments<<<gb, BT, 0, s>>>(L, b.st, b.P, b.g, cap, b.owner, b.ccount, b.cfirst,
+                                              b.wid_int, b.size2, b.rec64, r32, b.dip64, d32,
+                                              b.n_rec, b.lvl_start, b.by_level);
+        k_level_moments_block<<<1, 1024, 0, s>>>(LVL_WIDE_HI, 0, b.st, b.P, b.g, cap, b.owner,
+                                                 b.ccount, b.cfirst, b.wid_int, b.size2, b.rec64,
+                                                 r32, b.dip64, d32, b.n_rec, b.lvl_start,
+                                                 b.by_level);
     }
 }
 
