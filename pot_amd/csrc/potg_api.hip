@@ -322,6 +322,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_tree = (uint32_t)n_tree;
     b.n_inst = n_inst;
     b.precision = precision;
+    for (uint32_t i = 0; i < n_inst; i++) b.any_spheroid |= configs[i].geometry != POTG_SPHERE;
     int ib = 0;
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;

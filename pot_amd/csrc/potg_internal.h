@@ -20,6 +20,7 @@ struct Bufs {
     uint32_t n_tree;   // leading sorted slots that belong to the trees
     uint32_t n_inst;
     int precision;     // POTG_FP64 / POTG_MIXED / POTG_MIXED_GROUP
+    int any_spheroid;  // some instance has the analytic spheroid grain
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
     uint32_t walk_blocks; // blocks of the persistent walk grid

@@ -251,7 +251,10 @@ __device__ inline uint32_t n_int_of(const Globals* g, uint32_t cap)
     return n < cap ? n : cap;
 }
 
-// start of the level-L node that contains sorted slot p
+// start of the level-L node that contains sorted slot p: the first slot in
+// [tb, p] whose key is not below p's key prefix.  A backward galloping
+// search brackets it, then bisection: the cells of the deep levels, where
+// most nodes are, hold few slots.
 __device__ inline uint32_t range_start(const uint64_t* key, const int32_t* lcp, uint32_t tb,
                                        uint32_t p, int L, int levels)
 {
@@ -260,6 +263,14 @@ __device__ inline uint32_t range_start(const uint64_t* key, const int32_t* lcp, 
         const uint64_t low = (sh >= 64) ? ~0ull : ((1ull << sh) - 1ull);
         const uint64_t pk = key[p] & ~low;
         uint32_t lo = tb, hi = p;
+        for (uint64_t step = 1; step <= (uint64_t)(p - tb); step <<= 1) {
+            const uint32_t j = p - (uint32_t)step;
+            if (key[j] < pk) {
+                lo = j + 1;
+                break;
+            }
+            hi = j;
+        }
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
             if (key[mid] < pk) lo = mid + 1; else hi = mid;

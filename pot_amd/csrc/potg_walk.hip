@@ -47,7 +47,8 @@ struct WalkIO {
     const uint32_t* pt_inst;
     const uint32_t* pt_self;
     uint32_t n_pts;
-    double* out;       // in: grain term (initial accumulator); out: result
+    double* out;       // out: result; in, when acc is set: the grain term to add to
+    int32_t acc;       // 0: the walk starts from zero and never reads out
     uint32_t* counts;
     uint32_t* tile_ctr;
     uint32_t n_rec;    // record capacity: child ranges must stay inside it
@@ -155,11 +156,11 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
         const double theta2 = P[inst].theta2, soft2 = P[inst].soft2;
         double ex = 0.0, ey = 0.0, ez = 0.0, ph = 0.0;
         if (valid) {
-            if (KIND == KIND_FIELD) {
+            if (io.acc && KIND == KIND_FIELD) {
                 ex = io.out[3 * k];
                 ey = io.out[3 * k + 1];
                 ez = io.out[3 * k + 2];
-            } else {
+            } else if (io.acc) {
                 ph = io.out[k];
             }
         }
@@ -613,9 +614,9 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         }
         if (valid) {
             const double K = QE / (K4PE0 * p.R * p.R);
-            io.out[3 * k] += K * ((double)fx - (double)kx);
-            io.out[3 * k + 1] += K * ((double)fy - (double)ky);
-            io.out[3 * k + 2] += K * ((double)fz - (double)kz);
+            io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + K * ((double)fx - (double)kx);
+            io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + K * ((double)fy - (double)ky);
+            io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + K * ((double)fz - (double)kz);
             if (io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
@@ -780,9 +781,9 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
         }
         if (valid && slot == 0) {
             const double Kf = QE / (K4PE0 * p.R * p.R);
-            io.out[3 * k] += Kf * ex;
-            io.out[3 * k + 1] += Kf * ey;
-            io.out[3 * k + 2] += Kf * ez;
+            io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + Kf * ex;
+            io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + Kf * ey;
+            io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + Kf * ez;
             if (io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
@@ -1036,9 +1037,9 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
         }
         if (valid) {
             const double Kf = QE / (K4PE0 * p.R * p.R);
-            io.out[3 * k] += Kf * ((double)fx - (double)kx);
-            io.out[3 * k + 1] += Kf * ((double)fy - (double)ky);
-            io.out[3 * k + 2] += Kf * ((double)fz - (double)kz);
+            io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + Kf * ((double)fx - (double)kx);
+            io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + Kf * ((double)fy - (double)ky);
+            io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + Kf * ((double)fz - (double)kz);
             if (io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
@@ -1122,6 +1123,7 @@ static WalkIO step_io(Bufs& b)
     io.targets = b.targets;
     io.n_targets_dev = &b.g->n_targets;
     io.out = b.E;
+    io.acc = b.any_spheroid;
     io.tile_ctr = &b.g->tile_ctr;
     io.n_rec = b.n_rec;
     io.dip64 = b.dip64;
@@ -1131,6 +1133,8 @@ static WalkIO step_io(Bufs& b)
 
 void launch_walk_init(Bufs& b, hipStream_t s)
 {
+    // only the spheroids' analytic grain has a term to start from
+    if (!b.any_spheroid) return;
     WalkIO io = step_io(b);
     const uint32_t nt = b.n_tree ? b.n_tree : 1;
     k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
@@ -1187,6 +1191,7 @@ void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s)
     io.pt_self = b.q_self;
     io.n_pts = n;
     io.out = b.q_out;
+    io.acc = 1;
     io.counts = kind == 0 ? b.q_counts : nullptr;
     io.tile_ctr = &b.g->tile_ctr2;
     io.n_rec = b.n_rec;
@@ -1247,6 +1252,7 @@ void launch_epe(Bufs& b, hipStream_t s)
     WalkIO io{};
     io.n_pts = b.n_total;
     io.out = b.phi;
+    io.acc = 1;
     io.tile_ctr = &b.g->tile_ctr2;
     io.n_rec = b.n_rec;
     io.S = b.S;
