@@ -4,8 +4,9 @@
 // drift-kick-rotate-kick-drift in float64 (move_particles, pot.c:2472-2532),
 // the grain-crossing test (pot.c:2641-2685 with does_particle_cross_grain,
 // pot.c:2287-2325 / SPH:2448-2540), exact integer tallies of the absorbed
-// charge and counts, the boundary test (pot.c:2071-2078) and, inline for the
-// few lanes that need it, SCEPTIC-like reinjection (pot.c:496-652).
+// charge and counts and the boundary test (pot.c:2071-2078); then, a
+// wavefront per particle, SCEPTIC-like reinjection (pot.c:496-652) of the
+// few that were absorbed or left.
 // Reinjection draws come from a counter-based Philox4x32-10 stream keyed by
 // (seed, particle, step), so the result does not depend on thread order.
 #include "potg_grain.h"
@@ -110,30 +111,67 @@ __device__ double u_icdf(double U, double chi, double q)
     return u;
 }
 
+// lane j's double, in every lane
+__device__ inline double readlane_d(double v, int j)
+{
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), j);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// integrate_for_alpha (pot.c:433-492) by a whole wavefront, every lane
+// with the same arguments and the same result.  The serial loop bisects a
+// panel [xi0, xi2] until the integrand is linear within max_lin_dev at its
+// midpoint: level l's midpoint is the previous level's halved towards xi0,
+// and its test reads the integrand there and at the previous level.  Here
+// lanes 0..IFA_LV-1 evaluate IFA_LV consecutive levels at once and the
+// first level that passes is the panel; the midpoints, the integrand, the
+// tests and the sum are the serial loop's float64 operations in its order,
+// so the result is the serial one bit for bit.  The serial loop's guard of
+// 4000000 bisections in all is kept.
+constexpr int IFA_LV = 16;
+
 __device__ double integrate_for_alpha(double s, double chi_b, double u2, double Rol,
                                       double max_lin_dev)
 {
+    const int lane = (int)__lane_id();
+    const int kl = lane < IFA_LV ? lane : IFA_LV - 1;
     const double chu2 = chi_b / u2;
     const double s2 = s * s;
     const double y_one = s / sqrt(1.0 - chu2 - s2);
     double alpha = 0.0;
-    double xi0 = 0.0, xi1, xi2 = 1.0;
+    double xi0 = 0.0, xi2 = 1.0;
     double y0 = s, y1, y2;
-    int guard = 0;
+    long guard = 0;
     if (s == 0.0) return 0.0;
     do {
-        xi2 = 1.0;
-        y2 = y_one;
-        xi1 = (xi0 + xi2) / 2.0;
-        y1 = 1.0 - (chu2 * xi1 * exp(Rol * (1.0 - (1.0 / xi1))));
-        y1 = 1.0 / sqrt((y1 / s2) - (xi1 * xi1));
-        while (fabs(y2 - y1 - (y1 - y0)) > max_lin_dev) {
-            xi2 = xi1;
-            y2 = y1;
-            xi1 = (xi0 + xi2) / 2.0;
-            y1 = 1.0 - (chu2 * xi1 * exp(Rol * (1.0 - (1.0 / xi1))));
-            y1 = 1.0 / sqrt((y1 / s2) - (xi1 * xi1));
-            if (++guard > 4000000) return __builtin_nan("");
+        // levels base .. base + IFA_LV - 1 below the panel top (top, ytop)
+        double top = 1.0, ytop = y_one;
+        for (long base = 0;; base += IFA_LV) {
+            double xi1 = top;
+            for (int j = 0; j <= kl; j++) xi1 = (xi0 + xi1) / 2.0;
+            double y = 1.0 - (chu2 * xi1 * exp(Rol * (1.0 - (1.0 / xi1))));
+            y = 1.0 / sqrt((y / s2) - (xi1 * xi1));
+            const double xu = __shfl(xi1, lane > 0 ? lane - 1 : 0, WAVE);
+            const double yu = __shfl(y, lane > 0 ? lane - 1 : 0, WAVE);
+            const double xa = lane == 0 ? top : xu;   // the level above
+            const double ya = lane == 0 ? ytop : yu;
+            const uint64_t pass =
+                ballot(lane < IFA_LV && !(fabs(ya - y - (y - y0)) > max_lin_dev));
+            if (pass) {
+                const int k = __ffsll((unsigned long long)pass) - 1;
+                // level base + k took base + k bisections
+                guard += base + k;
+                if (guard > 4000000) return __builtin_nan("");
+                xi2 = readlane_d(xa, k);
+                y2 = readlane_d(ya, k);
+                y1 = readlane_d(y, k);
+                break;
+            }
+            if (guard + base + IFA_LV - 1 > 4000000) return __builtin_nan("");
+            top = readlane_d(xi1, IFA_LV - 1);
+            ytop = readlane_d(y, IFA_LV - 1);
         }
         alpha += (xi2 - xi0) * (y0 + (4.0 * y1) + y2) / 6.0;
         xi0 = xi2;
@@ -468,7 +506,7 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
         // grain collection (pot.c:2641-2685) and the boundary test
         // (pot.c:2071-2078) or the bounce off the wall (pot.c:2036-2069);
         // the rare reinjections go to a queue so that the long SCEPTIC
-        // sampling runs in lanes of its own (k_reinject)
+        // sampling runs in wavefronts of its own (k_reinject)
         int fate = 0;  // 1: absorbed, 2: left the domain
         if (p.grain_enabled && ((double)t * p.dt) > p.settle) {
             const double d[3] = {x - x0, y - y0, z - z0};
@@ -520,14 +558,18 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
 // particle is reinjected and then meets the boundary test like any other
 // (pot.c:2676, 2693); one that left the domain is reinjected once.  The
 // draws come from the particle's (seed, index, step) stream, so the order
-// of the queue does not matter.
+// of the queue does not matter.  One wavefront per queued particle: the
+// lanes repeat the sampling in step (same values in every lane) and share
+// the integration; lane 0 stores.
 __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ P, InstState* S,
                                                   Globals* g, PState st,
                                                   const uint32_t* __restrict__ queue,
                                                   const double* __restrict__ qprev, uint32_t qcap)
 {
     const uint32_t n = g->n_reinj < qcap ? g->n_reinj : qcap;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const uint32_t lane = __lane_id();
+    const uint32_t nw = gridDim.x * (BT / WAVE);
+    for (uint32_t e = blockIdx.x * (BT / WAVE) + threadIdx.x / WAVE; e < n; e += nw) {
         const uint32_t code = queue[e];
         const uint32_t i = code >> 1;
         const uint32_t inst = st.inst[i];
@@ -543,23 +585,26 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
             // the absorbed particle's start position is where a bounce
             // returns it (pot.c:2686-2690)
             if (bounce_off_wall(p, x, y, z, vx, vy, vz, qprev[3 * (size_t)e],
-                                qprev[3 * (size_t)e + 1], qprev[3 * (size_t)e + 2]) < 0)
+                                qprev[3 * (size_t)e + 1], qprev[3 * (size_t)e + 2]) < 0 &&
+                lane == 0)
                 atomicAdd(&S[inst].n_wall_inward, 1u);
         } else if (code & 1u) {
             const double rad = p.R;
             const double dx = x - rad, dy = y - rad, dz = z - rad;
             if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
-                atomicAdd(&S[inst].n_reinj_out, 1u);
+                if (lane == 0) atomicAdd(&S[inst].n_reinj_out, 1u);
                 if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
                 else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
             }
         }
-        st.x[i] = x;
-        st.y[i] = y;
-        st.z[i] = z;
-        st.vx[i] = vx;
-        st.vy[i] = vy;
-        st.vz[i] = vz;
+        if (lane == 0) {
+            st.x[i] = x;
+            st.y[i] = y;
+            st.z[i] = z;
+            st.vx[i] = vx;
+            st.vy[i] = vy;
+            st.vz[i] = vz;
+        }
     }
 }
 
@@ -780,7 +825,7 @@ void launch_push(Bufs& b, hipStream_t s)
 {
     k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.a1, b.x0, b.queue,
                                          b.qprev, b.n_tree);
-    k_reinject<<<64, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.qprev, b.n_tree);
+    k_reinject<<<256, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.qprev, b.n_tree);
 }
 
 void launch_half(Bufs& b, hipStream_t s)
