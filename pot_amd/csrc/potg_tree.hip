@@ -591,8 +591,16 @@ __device__ void level_node_moments(uint32_t k, PState st, const InstParams* __re
     double cx = 0.0, cy = 0.0, cz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     int q = 0;
     uint32_t aq = 0;
-    for (uint32_t j = first; j < end; j++) {
-        const NodeRec64 c = r64[j];
+    // every child's record in flight at once, then the serial fold
+    NodeRec64 ch[8];
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++)
+        if (first + j < end) ch[j] = r64[first + j];
+#pragma unroll
+    for (uint32_t jj = 0; jj < 8; jj++) {
+        if (first + jj >= end) break;
+        const uint32_t j = first + jj;
+        const NodeRec64& c = ch[jj];
         const int qc = c.q;
         const uint32_t aqc = c.cnt == LEAF ? (uint32_t)abs(qc) : c.part;
         if (aqc) {
