@@ -53,6 +53,23 @@ def test_tree_bit_exact(case):
         assert np.array_equal(ctx.tree_dipoles(0), sim.tree_dipoles())
 
 
+@pytest.mark.parametrize("N,geometry", [(4096, 0), (4097, 0), (12289, 0), (4097, 1), (8194, 2)])
+def test_tree_bit_exact_at_sort_tile_edges(N, geometry):
+    """The radix sort works on tiles of 4096 keys: trees of exactly one tile,
+    one key past it and three tiles and one key, bit for bit; with a
+    spheroid the grain's key, which sorts last, is alone in the last tile
+    (4097) or shares it with one particle (8194)."""
+    case = big_case(N, geometry=geometry, A=2.0 if geometry == 1 else 0.5, seed=N + geometry)
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    sim.build_tree()
+    ctx = gpu_ctx(case, parts, precision=G.POTG_MIXED)
+    ctx.build_tree()
+    got, ref = ctx.tree_canon(0), sim.tree_canon()
+    assert len(got["depth"]) == len(ref["depth"])
+    canon_equal(got, ref)
+
+
 @pytest.mark.parametrize("case", ALL, ids=IDS)
 def test_walk_fp64_matches_oracle(case):
     sim = oracle_sim(case)
