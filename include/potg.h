@@ -220,6 +220,11 @@ int potg_set_tuning(potg_ctx* ctx, const potg_tuning* set, potg_tuning* get);
 
 int potg_stats(potg_ctx* ctx, potg_stats_t* out);
 int potg_stats_reset(potg_ctx* ctx);
+/* Interaction counters (n_cell, n_leaf, n_open, n_range_loads) of the step
+   walks on (on != 0, the default) or off.  They are instrumentation: off,
+   the float32 walk runs about 4% faster; point queries count either way.
+   No counterpart in the reference. */
+int potg_set_counting(potg_ctx* ctx, int32_t on);
 
 /* Per-stage wall time (ms) averaged over the potg_step calls since timing
    was last enabled, from HIP events on the context's stream: [0] keys +

@@ -323,6 +323,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_inst = n_inst;
     b.precision = precision;
     for (uint32_t i = 0; i < n_inst; i++) b.any_spheroid |= configs[i].geometry != POTG_SPHERE;
+    b.count = 1;
     int ib = 0;
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
@@ -374,7 +375,6 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     rc |= alloc_n(c, &b.owner, ci) | alloc_n(c, &b.level, ci);
     rc |= alloc_n(c, &b.slots, 8 * ci) | alloc_n(c, &b.ccount, ci) | alloc_n(c, &b.cfirst, ci);
     rc |= alloc_n(c, &b.wid_int, ci) | alloc_n(c, &b.wid_leaf, nt);
-    rc |= alloc_n(c, &b.par_int, ci) | alloc_n(c, &b.par_leaf, nt) | alloc_n(c, &b.arrive, ci);
     rc |= alloc_n(c, &b.size2, ci);
     rc |= alloc_n(c, &b.lvl_hist, (size_t)MAXD * LVL_BLOCKS);
     rc |= alloc_n(c, &b.lvl_off, (size_t)MAXD * LVL_BLOCKS);
@@ -834,6 +834,14 @@ int potg_stats(potg_ctx* c, potg_stats_t* out)
     out->n_particles = c->b.n_total;
     out->max_depth = g.max_depth;
     out->n_range_loads = g.n_pops;
+    return POTG_OK;
+}
+
+int potg_set_counting(potg_ctx* c, int32_t on)
+{
+    if (!c) return set_err("potg_set_counting: null context"), POTG_ERR_ARG;
+    if ((on != 0) != (c->b.count != 0)) destroy_graphs(c);  // captured with the old kernels
+    c->b.count = on != 0;
     return POTG_OK;
 }
 

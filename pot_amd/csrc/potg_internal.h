@@ -21,6 +21,7 @@ struct Bufs {
     uint32_t n_inst;
     int precision;     // POTG_FP64 / POTG_MIXED / POTG_MIXED_GROUP
     int any_spheroid;  // some instance has the analytic spheroid grain
+    int count;         // step walks keep the interaction counters (potg_set_counting)
     int key_levels;    // digits carried by a sort key
     int key_bits;      // bits of the key that are sorted
     uint32_t walk_blocks; // blocks of the persistent walk grid
@@ -44,9 +45,6 @@ struct Bufs {
     uint32_t* cfirst;
     uint32_t* wid_int;
     uint32_t* wid_leaf;
-    uint32_t* par_int;
-    uint32_t* par_leaf;
-    uint32_t* arrive;
     uint32_t *lvl_hist, *lvl_off;  // per-block depth histograms and their scan
     uint32_t* lvl_start;           // first slot of each depth in by_level
     uint32_t* by_level;  // internal node ids grouped by depth

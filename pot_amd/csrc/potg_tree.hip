@@ -225,7 +225,7 @@ __global__ void k_counts(const int32_t* __restrict__ lcp, uint32_t n_tree, uint3
 __global__ void k_owners(const int32_t* __restrict__ lcp, const uint32_t* __restrict__ base,
                          uint32_t n_tree, uint32_t cap, uint32_t* __restrict__ owner,
                          int32_t* __restrict__ level, uint32_t* __restrict__ slots,
-                         uint32_t* __restrict__ arrive, Globals* g)
+                         Globals* g)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_tree) return;
@@ -239,7 +239,6 @@ __global__ void k_owners(const int32_t* __restrict__ lcp, const uint32_t* __rest
         }
         owner[k] = i;
         level[k] = l;
-        arrive[k] = 0;
 #pragma unroll
         for (int s = 0; s < 8; s++) slots[8 * k + s] = NONE;
     }
@@ -296,14 +295,14 @@ __device__ inline int digit_of(const uint64_t* key, PState st, uint32_t i, int l
     return descend_to(st.x[i], st.y[i], st.z[i], root_x2, l, x1, x2);
 }
 
-// parents, digits and child slots of internal nodes (k < n_internal) and
-// of leaves (every tree slot)
+// every internal node (k < n_internal) and every leaf (tree slot) enters
+// its parent's child slot at its digit; internal nodes also get their box
+// size, roots their walk id
 __global__ void k_link(const uint64_t* __restrict__ key, const int32_t* __restrict__ lcp,
                        const uint32_t* __restrict__ base, const uint32_t* __restrict__ owner,
                        const int32_t* __restrict__ level, PState st,
                        const InstParams* __restrict__ P, uint32_t n_tree, int levels,
                        uint32_t cap, Globals* g, uint32_t* __restrict__ slots,
-                       uint32_t* __restrict__ par_int, uint32_t* __restrict__ par_leaf,
                        uint32_t* __restrict__ wid_int, uint32_t* __restrict__ wid_leaf,
                        double* __restrict__ size2)
 {
@@ -318,7 +317,6 @@ __global__ void k_link(const uint64_t* __restrict__ key, const int32_t* __restri
         const int d = descend_to(st.x[a], st.y[a], st.z[a], p.root_x2, l, x1, x2);
         size2[k] = (x2 - x1) * (x2 - x1);
         if (l == 0) {
-            par_int[k] = NONE;
             wid_int[k] = inst;
         } else {
             const int lo = (a == 0) ? -1 : lcp[a - 1];
@@ -329,13 +327,8 @@ __global__ void k_link(const uint64_t* __restrict__ key, const int32_t* __restri
                 const uint32_t a2 = range_start(key, lcp, p.tree_begin, a, l - 1, levels);
                 par = node_id(lcp, base, a2, l - 1);
             }
-            if (par >= nint) {
-                raise_err(g, KERR_CAPACITY, k);
-                par_int[k] = NONE;
-            } else {
-                par_int[k] = par;
-                slots[8 * par + d] = k;
-            }
+            if (par >= nint) raise_err(g, KERR_CAPACITY, k);
+            else slots[8 * par + d] = k;
         }
     }
     if (k < n_tree) {
@@ -346,8 +339,7 @@ __global__ void k_link(const uint64_t* __restrict__ key, const int32_t* __restri
         const int hi = lcp[i];
         const int L = hi > lo ? hi : lo;
         if (L < 0) {
-            par_leaf[i] = NONE;  // a one-particle tree: the leaf is the root
-            wid_leaf[i] = inst;
+            wid_leaf[i] = inst;  // a one-particle tree: the leaf is the root
         } else {
             uint32_t par;
             if (hi > lo) {
@@ -358,10 +350,8 @@ __global__ void k_link(const uint64_t* __restrict__ key, const int32_t* __restri
             }
             if (par >= nint) {
                 raise_err(g, KERR_CAPACITY, i);
-                par_leaf[i] = NONE;
                 wid_leaf[i] = inst;
             } else {
-                par_leaf[i] = par;
                 slots[8 * par + digit_of(key, st, i, L + 1, levels, p.root_x2)] = LEAF_BIT | i;
             }
         }
@@ -730,12 +720,10 @@ void launch_tree(Bufs& b, hipStream_t s)
         k_counts<<<grid(n), BT, 0, s>>>(b.lcp, n, b.cnt);
         exclusive_scan(b.cnt, b.base, n, nullptr, &b.g->n_internal, nullptr, b.scan_tmp, 0, s);
         const uint32_t cap = b.cap_int;
-        k_owners<<<grid(n), BT, 0, s>>>(b.lcp, b.base, n, cap, b.owner, b.level, b.slots, b.arrive,
-                                        b.g);
+        k_owners<<<grid(n), BT, 0, s>>>(b.lcp, b.base, n, cap, b.owner, b.level, b.slots, b.g);
         k_link<<<grid(cap > n ? cap : n), BT, 0, s>>>(b.key, b.lcp, b.base, b.owner, b.level, b.st,
                                                        b.P, n, b.key_levels, cap, b.g, b.slots,
-                                                       b.par_int, b.par_leaf, b.wid_int, b.wid_leaf,
-                                                       b.size2);
+                                                       b.wid_int, b.wid_leaf, b.size2);
         k_ccount<<<grid(cap), BT, 0, s>>>(b.slots, b.g, cap, b.ccount);
         exclusive_scan(b.ccount, b.cfirst, cap, &b.g->n_internal, &b.g->n_children, nullptr,
                        b.scan_tmp, b.n_inst, s);

@@ -131,7 +131,7 @@ __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, 
 }
 
 // ---- float64 walk (field or potential) --------------------------------
-template <int MODE, int KIND, bool DIP>
+template <int MODE, int KIND, bool DIP, bool COUNT>
 __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec64* __restrict__ rec,
                                                 WalkIO io)
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
             } else {
                 io.out[k] = ph;
             }
-            if (io.counts) {
+            if (COUNT && io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
                 io.counts[3 * k + 2] = nopen;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
         tot_leaf += nleaf;
         tot_open += nopen;
     }
-    if (KIND == KIND_FIELD) {
+    if (COUNT && KIND == KIND_FIELD) {
         for (int d = 1; d < WAVE; d <<= 1) {
             tot_cell += __shfl_xor(tot_cell, d, WAVE);
             tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
@@ -357,8 +357,9 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
 // the same for all lanes (leaf or cell, which child next) is scalar, so a
 // child step costs few scalar instructions besides its vector arithmetic;
 // lanes not taking a child get zero coefficients and every lane adds
-// a * d + b * p to its sums in place.  The wavefront's interaction totals
-// are scalar popcounts; per-lane counts are kept only for point queries.
+// a * d + b * p to its sums in place.  Interaction counts, when kept, are
+// per-lane counters summed once per kernel: a scalar popcount per child
+// cost the walk twice as much (the scalar unit is shared by the CU).
 // one (target, node) interaction; returns true when the target opens the node.
 // With dip (DIPOLES), cells holding any charge are accepted and pd is the
 // cell's dipole in units of R (pot.c:1732-1745).
@@ -423,7 +424,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
                                   uint32_t hqm, uint64_t tm, float tx, float ty, float tz,
                                   uint32_t self, float theta2, float soft2, float& px, float& py,
                                   float& pz, uint32_t& ncell, uint32_t& nleaf, uint32_t& nopen,
-                                  uint32_t& wcell, uint32_t& wleaf, uint32_t& next_first,
+                                  uint32_t& next_first,
                                   uint32_t& next_cnt, uint64_t& next_mask)
 {
     const uint32_t lane = __lane_id();
@@ -442,7 +443,6 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
         px = __builtin_fmaf(ca, dx, px);
         py = __builtin_fmaf(ca, dy, py);
         pz = __builtin_fmaf(ca, dz, pz);
-        wleaf += (uint32_t)__popcll(tm & ballot(pn) & ballot(dn));
         if (CNT) nleaf += use ? 1u : 0u;
     }
     for (uint32_t c = rem & ~lm; c;) {
@@ -474,10 +474,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
             py = __builtin_fmaf(ca, dy, py);
             pz = __builtin_fmaf(ca, dz, pz);
         }
-        if ((hqm >> j) & 1u) {
-            wcell += (uint32_t)__popcll(tm & am);
-            if (CNT) ncell += use ? 1u : 0u;
-        }
+        if (CNT && ((hqm >> j) & 1u)) ncell += use ? 1u : 0u;
         const uint64_t om = tm & ~am;
         if (om) {
             if (CNT && ((om >> lane) & 1ull)) nopen++;
@@ -492,7 +489,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
     return -1;
 }
 
-template <int MODE, bool DIP>
+template <int MODE, bool DIP, bool COUNT>
 __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P, Globals* g,
                                                 PState st, const NodeRec32* __restrict__ rec,
                                                 WalkIO io)
@@ -505,8 +502,9 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
     const int wv = threadIdx.x / WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
-    // per-lane counts: the point queries only
-    constexpr bool CNT = MODE == MODE_POINTS;
+    // per-lane interaction counts: for the point queries' own counts, and
+    // for the context's totals when counting
+    constexpr bool CNT = MODE == MODE_POINTS || COUNT;
     unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
 
     for (;;) {
@@ -526,8 +524,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         const float soft2 = (float)(p.soft2 * p.invR * p.invR);
         float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         uint32_t ncell = 0, nleaf = 0, nopen = 0;
-        // the tile's totals as scalars (a tile's counts fit 32 bits)
-        uint32_t wcell = 0, wleaf = 0, wopen = 0;
+        uint32_t wopen = 0;  // lanes descending, as a scalar total
 
         uint64_t pending = ballot(valid);
         while (pending) {
@@ -580,12 +577,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     (DIP && dip)
                         ? visit_range<true, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
                                                  tz, self, theta2, soft2, px, py, pz, ncell, nleaf,
-                                                 nopen, wcell, wleaf, next_first, next_cnt,
-                                                 next_mask)
+                                                 nopen, next_first, next_cnt, next_mask)
                         : visit_range<false, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
                                                   tz, self, theta2, soft2, px, py, pz, ncell,
-                                                  nleaf, nopen, wcell, wleaf, next_first, next_cnt,
-                                                  next_mask);
+                                                  nleaf, nopen, next_first, next_cnt, next_mask);
                 kahan_add(fx, kx, px);
                 kahan_add(fy, ky, py);
                 kahan_add(fz, kz, pz);
@@ -617,23 +612,27 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
             io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + K * ((double)fx - (double)kx);
             io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + K * ((double)fy - (double)ky);
             io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + K * ((double)fz - (double)kz);
-            if (io.counts) {
+            if (COUNT && io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
                 io.counts[3 * k + 2] = nopen;
             }
         }
-        if (lane == 0) {
-            tot_cell += wcell;
-            tot_leaf += wleaf;
-            tot_open += wopen;
-        }
+        tot_cell += ncell;
+        tot_leaf += nleaf;
+        if (lane == 0) tot_open += wopen;
     }
-    if (lane == 0) {
-        atomicAdd(&g->n_cell, tot_cell);
-        atomicAdd(&g->n_leaf, tot_leaf);
-        atomicAdd(&g->n_open, tot_open);
-        atomicAdd(&g->n_pops, tot_pops);
+    if (COUNT) {
+        for (int d = 1; d < WAVE; d <<= 1) {
+            tot_cell += __shfl_xor(tot_cell, d, WAVE);
+            tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+        }
+        if (lane == 0) {
+            atomicAdd(&g->n_cell, tot_cell);
+            atomicAdd(&g->n_leaf, tot_leaf);
+            atomicAdd(&g->n_open, tot_open);
+            atomicAdd(&g->n_pops, tot_pops);
+        }
     }
 }
 
@@ -654,7 +653,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
 // Each lane sums its own (target, slot) contributions in float32, flushed
 // every 4 ranges into a compensated sum; at the end the slots of a target
 // are added in float64 across lanes.
-template <int K, int MODE, bool DIP>
+template <int K, int MODE, bool DIP, bool COUNT>
 __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, Globals* g,
                                                PState st, const NodeRec32* __restrict__ rec,
                                                WalkIO io)
@@ -784,7 +783,7 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
             io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + Kf * ex;
             io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + Kf * ey;
             io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + Kf * ez;
-            if (io.counts) {
+            if (COUNT && io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
                 io.counts[3 * k + 2] = nopen;
@@ -796,16 +795,18 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
             tot_open += nopen;
         }
     }
-    for (int d = 1; d < WAVE; d <<= 1) {
-        tot_cell += __shfl_xor(tot_cell, d, WAVE);
-        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
-        tot_open += __shfl_xor(tot_open, d, WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd(&g->n_cell, tot_cell);
-        atomicAdd(&g->n_leaf, tot_leaf);
-        atomicAdd(&g->n_open, tot_open);
-        atomicAdd(&g->n_pops, tot_pops);
+    if (COUNT) {
+        for (int d = 1; d < WAVE; d <<= 1) {
+            tot_cell += __shfl_xor(tot_cell, d, WAVE);
+            tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+            tot_open += __shfl_xor(tot_open, d, WAVE);
+        }
+        if (lane == 0) {
+            atomicAdd(&g->n_cell, tot_cell);
+            atomicAdd(&g->n_leaf, tot_leaf);
+            atomicAdd(&g->n_open, tot_open);
+            atomicAdd(&g->n_pops, tot_pops);
+        }
     }
 }
 
@@ -832,7 +833,7 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
 // fills or the walk ends, runs the list: every lane adds the interaction of
 // each entry with its own target, the entry's record coming in through
 // scalar loads.  The target's own leaf is skipped (pot.c:1787).
-template <int MODE, bool DIP>
+template <int MODE, bool DIP, bool COUNT>
 __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__ P, Globals* g,
                                                    PState st, const NodeRec32* __restrict__ rec,
                                                    WalkIO io)
@@ -1040,7 +1041,7 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
             io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + Kf * ((double)fx - (double)kx);
             io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + Kf * ((double)fy - (double)ky);
             io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + Kf * ((double)fz - (double)kz);
-            if (io.counts) {
+            if (COUNT && io.counts) {
                 io.counts[3 * k] = ncell;
                 io.counts[3 * k + 1] = nleaf;
                 io.counts[3 * k + 2] = nopen;
@@ -1050,16 +1051,18 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
         tot_leaf += valid ? nleaf : 0u;
         tot_open += lane == 0 ? nopen : 0u;
     }
-    for (int d = 1; d < WAVE; d <<= 1) {
-        tot_cell += __shfl_xor(tot_cell, d, WAVE);
-        tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
-        tot_open += __shfl_xor(tot_open, d, WAVE);
-    }
-    if (lane == 0) {
-        atomicAdd(&g->n_cell, tot_cell);
-        atomicAdd(&g->n_leaf, tot_leaf);
-        atomicAdd(&g->n_open, tot_open);
-        atomicAdd(&g->n_pops, tot_pops);
+    if (COUNT) {
+        for (int d = 1; d < WAVE; d <<= 1) {
+            tot_cell += __shfl_xor(tot_cell, d, WAVE);
+            tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+            tot_open += __shfl_xor(tot_open, d, WAVE);
+        }
+        if (lane == 0) {
+            atomicAdd(&g->n_cell, tot_cell);
+            atomicAdd(&g->n_leaf, tot_leaf);
+            atomicAdd(&g->n_open, tot_open);
+            atomicAdd(&g->n_pops, tot_pops);
+        }
     }
 }
 
@@ -1143,35 +1146,50 @@ void launch_walk_init(Bufs& b, hipStream_t s)
 // float32 field walk of the chosen shape: the group criterion
 // (k_walk32grp), groups of 8 or 16 targets (k_walkg), or 64 targets stepping
 // through the children (k_walk32); instantiated with dipole support only
-// for contexts holding DIPOLES instances
-template <int MODE>
-static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
+// for contexts holding DIPOLES instances, and with interaction counting
+// (COUNT) for point queries and for step walks while the context counts
+template <int MODE, bool COUNT>
+static void launch_walk32c(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
 {
     const bool dip = b.dip32 != nullptr;
 #define POTG_W32(KERNEL) KERNEL<<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec32, io)
     if (b.precision == POTG_MIXED_GROUP) {
-        if (dip) POTG_W32((k_walk32grp<MODE, true>));
-        else POTG_W32((k_walk32grp<MODE, false>));
+        if (dip) POTG_W32((k_walk32grp<MODE, true, COUNT>));
+        else POTG_W32((k_walk32grp<MODE, false, COUNT>));
     } else if (b.group == 8) {
-        if (dip) POTG_W32((k_walkg<8, MODE, true>));
-        else POTG_W32((k_walkg<8, MODE, false>));
+        if (dip) POTG_W32((k_walkg<8, MODE, true, COUNT>));
+        else POTG_W32((k_walkg<8, MODE, false, COUNT>));
     } else if (b.group == 16) {
-        if (dip) POTG_W32((k_walkg<16, MODE, true>));
-        else POTG_W32((k_walkg<16, MODE, false>));
+        if (dip) POTG_W32((k_walkg<16, MODE, true, COUNT>));
+        else POTG_W32((k_walkg<16, MODE, false, COUNT>));
     } else {
-        if (dip) POTG_W32((k_walk32<MODE, true>));
-        else POTG_W32((k_walk32<MODE, false>));
+        if (dip) POTG_W32((k_walk32<MODE, true, COUNT>));
+        else POTG_W32((k_walk32<MODE, false, COUNT>));
     }
 #undef POTG_W32
+}
+
+template <int MODE>
+static void launch_walk32(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
+{
+    if (MODE == MODE_POINTS || b.count) launch_walk32c<MODE, true>(b, blocks, io, s);
+    else launch_walk32c<MODE, false>(b, blocks, io, s);
+}
+
+template <int MODE, int KIND, bool COUNT>
+static void launch_walk64c(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
+{
+    if (b.dip64)
+        k_walk64<MODE, KIND, true, COUNT><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    else
+        k_walk64<MODE, KIND, false, COUNT><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
 }
 
 template <int MODE, int KIND>
 static void launch_walk64(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
 {
-    if (b.dip64)
-        k_walk64<MODE, KIND, true><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
-    else
-        k_walk64<MODE, KIND, false><<<blocks, BT, 0, s>>>(b.P, b.g, b.st, b.rec64, io);
+    if (MODE == MODE_POINTS || b.count) launch_walk64c<MODE, KIND, true>(b, blocks, io, s);
+    else launch_walk64c<MODE, KIND, false>(b, blocks, io, s);
 }
 
 void launch_walk(Bufs& b, hipStream_t s)

@@ -645,6 +645,7 @@ int main(int argc, char** argv)
         free(h.parts);
         return EXIT_FAILURE;
     }
+    potg_set_counting(h.ctx, 0);  /* the program reports no interaction counts */
     memset(&g, 0, sizeof(g));
     g.q = h.st.grain_q;
     g.m = (double)h.st.grain_m;

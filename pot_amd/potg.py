@@ -28,7 +28,7 @@ EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "pot
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
            "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles", "potg_tune",
-           "potg_set_tuning"]
+           "potg_set_tuning", "potg_set_counting"]
 
 
 class Config(C.Structure):
@@ -98,6 +98,7 @@ def lib():
     L.potg_direct_sum.argtypes = [P, C.c_uint32, up, C.c_size_t, dp]
     L.potg_stats.argtypes = [P, C.POINTER(Stats)]
     L.potg_stats_reset.argtypes = [P]
+    L.potg_set_counting.argtypes = [P, C.c_int32]
     L.potg_stage_times.argtypes = [P, C.c_int32, dp]
     L.potg_sync.argtypes = [P]
     L.potg_tree_canon_dipoles.argtypes = [P, C.c_uint32, P, C.c_int64, C.POINTER(C.c_int64)]
@@ -268,6 +269,9 @@ class Context:
 
     def stats_reset(self):
         _check(self.L.potg_stats_reset(self.h))
+
+    def set_counting(self, on):
+        _check(self.L.potg_set_counting(self.h, 1 if on else 0))
 
     def stage_times(self, enable=1):
         ms = np.zeros(5)

@@ -221,6 +221,45 @@ def test_tuning_takes_no_step_and_is_reproducible():
         assert a.stats()["steps"] == 4
 
 
+@pytest.mark.parametrize("precision,group", [(G.POTG_FP64, 64), (G.POTG_MIXED, 64),
+                                             (G.POTG_MIXED, 16), (G.POTG_MIXED_GROUP, 64)],
+                         ids=["fp64", "mixed64", "mixed16", "mixed_group"])
+@pytest.mark.parametrize("dipoles", [0, 1], ids=["monopole", "dipoles"])
+def test_counting_off_changes_only_the_counters(precision, group, dipoles):
+    """potg_set_counting(0) drops the interaction counters of the step walks
+    and nothing else: the same steps end in the same state bit for bit, the
+    step and target counts are kept, and point queries still count."""
+    case = dict(big_case(20001, seed=31), dipoles=dipoles)
+    parts = oracle_sim(case).particles()
+    a = gpu_ctx(case, parts, precision=precision, seed=5)
+    b = gpu_ctx(case, parts, precision=precision, seed=5)
+    for c in (a, b):
+        c.set_tuning(2048, group)
+    b.set_counting(False)
+    for _ in range(3):
+        a.step()
+        b.step()
+    pa, ga = a.download(0)
+    pb, gb = b.download(0)
+    assert np.array_equal(pa.view(np.uint8), pb.view(np.uint8))
+    assert ga == gb
+    sa, sb = a.stats(), b.stats()
+    assert sa["n_cell"] > 0 and sa["n_leaf"] > 0 and sa["n_open"] > 0
+    for k in ("n_cell", "n_leaf", "n_open", "n_range_loads"):
+        assert sb[k] == 0, k
+    for k in ("steps", "n_targets"):
+        assert sa[k] == sb[k], k
+    # a graph replay after switching counting back on counts again
+    b.set_counting(True)
+    b.run(2)
+    assert b.stats()["n_cell"] > 0
+    idx = np.arange(1, 2001, 7, dtype=np.uint32)
+    xyz = np.stack([pb["x"][idx], pb["y"][idx], pb["z"][idx]], 1)
+    b.set_counting(False)
+    _, cnt = b.field_at(0, xyz, exclude=idx, counts=True)
+    assert cnt[:, 1].min() > 0
+
+
 @pytest.mark.parametrize("group,blocks", [(64, 2048), (64, 300), (8, 2048), (16, 2048)])
 @pytest.mark.parametrize("case", [BIG[0], BIG[3]], ids=["monopole", "dipoles"])
 def test_mixed_walk_shapes_agree(case, group, blocks):
