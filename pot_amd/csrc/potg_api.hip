@@ -131,6 +131,7 @@ void fill_params(InstParams& p, const potg_config& k)
     p.dt = k.dt;
     p.soft2 = k.soft * k.soft;
     p.theta2 = k.theta * k.theta;
+    p.open32 = 1.0 / (p.theta2 * p.R * p.R);  // infinite for theta = 0: every cell opens
     p.a = k.a;
     p.A = k.A;
     p.settle = k.settle;

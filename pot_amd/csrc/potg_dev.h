@@ -45,6 +45,7 @@ struct InstParams {
     double dt;
     double soft2;      // soft * soft
     double theta2;     // OAP * OAP
+    double open32;     // 1 / (theta2 R^2): float32 records store size2 * open32
     double a, A;
     double settle;
     double x_drift, Te, Ti;
@@ -121,7 +122,8 @@ struct __align__(16) NodeRec64 {
     uint32_t part;  // leaf: sorted slot of its particle
 };
 struct __align__(16) NodeRec32 {
-    float cx, cy, cz, size2;   // relative to the domain centre, in units of R
+    float cx, cy, cz;          // relative to the domain centre, in units of R
+    float size2;               // box size squared over theta^2, in units of R^2
     float q;                   // net charge (exact: |q| < 2^24)
     uint32_t first;
     uint32_t cnt;

@@ -419,7 +419,9 @@ __device__ inline void put32(NodeRec32* r32, uint32_t w, const InstParams& p, do
     f.cx = (float)((cx - p.R) * p.invR);
     f.cy = (float)((cy - p.R) * p.invR);
     f.cz = (float)((cz - p.R) * p.invR);
-    f.size2 = (float)(s2 * p.invR * p.invR);
+    // the opening test of the float32 walks is size2 < d2: the box size
+    // squared over theta^2, in units of R^2 (with theta = 0 every cell opens)
+    f.size2 = s2 > 0.0 ? (float)(s2 * p.open32) : 0.f;
     f.q = (float)q;
     f.first = first;
     f.cnt = cnt;

@@ -363,7 +363,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
 // one (target, node) interaction; returns true when the target opens the node.
 // With dip (DIPOLES), cells holding any charge are accepted and pd is the
 // cell's dipole in units of R (pot.c:1732-1745).
-__device__ inline bool interact32(float tx, float ty, float tz, uint32_t self, float theta2,
+__device__ inline bool interact32(float tx, float ty, float tz, uint32_t self,
                                   float soft2, const NodeRec32& r, bool dip, float4 pd, float& ax,
                                   float& ay, float& az, uint32_t& ncell, uint32_t& nleaf)
 {
@@ -380,7 +380,7 @@ __device__ inline bool interact32(float tx, float ty, float tz, uint32_t self, f
         }
         return false;
     }
-    if (r.size2 < theta2 * d2) {
+    if (r.size2 < d2) {
         if (dip ? r.part != 0u : r.q != 0.f) {
             const float inv = __builtin_amdgcn_rsqf(d2);
             const float inv2 = inv * inv;
@@ -422,7 +422,7 @@ __device__ inline bool nonzero_bits(uint32_t b)
 template <bool D, bool CNT>
 __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_t& rem, uint32_t lm,
                                   uint32_t hqm, uint64_t tm, float tx, float ty, float tz,
-                                  uint32_t self, float theta2, float soft2, float& px, float& py,
+                                  uint32_t self, float soft2, float& px, float& py,
                                   float& pz, uint32_t& ncell, uint32_t& nleaf, uint32_t& nopen,
                                   uint32_t& next_first,
                                   uint32_t& next_cnt, uint64_t& next_mask)
@@ -451,7 +451,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
         const NodeRec32 r = sr[j];
         const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        const bool acc = r.size2 < theta2 * d2;
+        const bool acc = r.size2 < d2;
         const uint64_t am = ballot(acc);
         const bool use = act && acc;
         // lanes not taking the cell get zero coefficients (and nothing
@@ -520,7 +520,6 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         const float tx = (float)((dtx - p.R) * p.invR);
         const float ty = (float)((dty - p.R) * p.invR);
         const float tz = (float)((dtz - p.R) * p.invR);
-        const float theta2 = (float)p.theta2;
         const float soft2 = (float)(p.soft2 * p.invR * p.invR);
         float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         uint32_t ncell = 0, nleaf = 0, nopen = 0;
@@ -576,10 +575,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                 const int j =
                     (DIP && dip)
                         ? visit_range<true, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
-                                                 tz, self, theta2, soft2, px, py, pz, ncell, nleaf,
+                                                 tz, self, soft2, px, py, pz, ncell, nleaf,
                                                  nopen, next_first, next_cnt, next_mask)
                         : visit_range<false, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
-                                                  tz, self, theta2, soft2, px, py, pz, ncell,
+                                                  tz, self, soft2, px, py, pz, ncell,
                                                   nleaf, nopen, next_first, next_cnt, next_mask);
                 kahan_add(fx, kx, px);
                 kahan_add(fy, ky, py);
@@ -685,7 +684,6 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
         const float tx = (float)((dtx - p.R) * p.invR);
         const float ty = (float)((dty - p.R) * p.invR);
         const float tz = (float)((dtz - p.R) * p.invR);
-        const float theta2 = (float)p.theta2;
         const float soft2 = (float)(p.soft2 * p.invR * p.invR);
         float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         float px = 0.f, py = 0.f, pz = 0.f;
@@ -726,7 +724,7 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
                         const NodeRec32 r = rec[tf + c];
                         float4 pd = make_float4(0.f, 0.f, 0.f, 0.f);
                         if (DIP && dip) pd = io.dip32[tf + c];
-                        op = interact32(tx, ty, tz, self, theta2, soft2, r, DIP && dip, pd, px, py,
+                        op = interact32(tx, ty, tz, self, soft2, r, DIP && dip, pd, px, py,
                                         pz, ncell, nleaf);
                         cf = r.first;
                         cn = r.cnt;
@@ -888,7 +886,6 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
             const bool mine = valid && inst == gi && cell == gc;
             pending &= ~rfl64(ballot(mine));
             const bool dip = DIP && P[gi].dipoles && io.dip32;
-            const float theta2 = (float)P[gi].theta2;
             // bounding box of this instance's targets in the wavefront
             float lx = mine ? tx : 3.f, ly = mine ? ty : 3.f, lz = mine ? tz : 3.f;
             float hx = mine ? tx : -3.f, hy = mine ? ty : -3.f, hz = mine ? tz : -3.f;
@@ -949,7 +946,7 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
                             const float ey = fmaxf(0.f, fmaxf(ly - r.cy, r.cy - hy));
                             const float ez = fmaxf(0.f, fmaxf(lz - r.cz, r.cz - hz));
                             const float dmin2 = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, ez * ez));
-                            if (r.size2 < theta2 * dmin2)
+                            if (r.size2 < dmin2)
                                 to_list = dip ? r.part != 0u : r.q != 0.f;
                             else if (r.cnt != 0) {
                                 to_stack = true;
