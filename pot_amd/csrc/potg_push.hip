@@ -130,7 +130,7 @@ __device__ inline double readlane_d(double v, int j)
 // tests and the sum are the serial loop's float64 operations in its order,
 // so the result is the serial one bit for bit.  The serial loop's guard of
 // 4000000 bisections in all is kept.
-constexpr int IFA_LV = 16;
+constexpr int IFA_LV = 8;  // panels deeper than 7 levels are rare
 
 __device__ double integrate_for_alpha(double s, double chi_b, double u2, double Rol,
                                       double max_lin_dev)
