@@ -338,15 +338,19 @@ def test_group_walk_steps_and_tunes():
 @pytest.mark.parametrize("integrator,bounce", [(0, None), (1, None), (2, 0.95)],
                          ids=["boris", "verlet", "euler_richardson_bounce"])
 @pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
-def test_graph_replay_equals_eager_steps(integrator, bounce, precision):
+@pytest.mark.parametrize("counting", [True, False], ids=["counted", "uncounted"])
+def test_graph_replay_equals_eager_steps(integrator, bounce, precision, counting):
     """potg_run replays captured steps (two graphs alternating the state
     buffers); they must equal steps enqueued one by one, for every
-    integrator (the two-field ones capture a second walk) and both
-    precisions."""
+    integrator (the two-field ones capture a second walk), both precisions,
+    and with the interaction counters on or off (the batched sweep runs
+    uncounted graphs)."""
     case = dict(big_case(20001, seed=21), integrator=integrator, bounce=bounce)
     parts = oracle_sim(case).particles()
     a = gpu_ctx(case, parts, precision=precision, seed=5)
     b = gpu_ctx(case, parts, precision=precision, seed=5)
+    for c in (a, b):
+        c.set_counting(counting)
     for _ in range(6):
         a.step()
     b.run(6)
