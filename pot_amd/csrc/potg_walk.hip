@@ -419,8 +419,16 @@ __device__ inline bool nonzero_bits(uint32_t b)
 // cell's index (with its range and the opening lanes in next_*) and leaves
 // the cells still to visit in rem; -1 when the range is done.  hqm marks
 // the children holding charge (the reference skips the others, pot.c:1732).
-template <bool D, bool CNT>
-__device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_t& rem, uint32_t lm,
+// a staged child: its record, then (DIPOLES kernels) its dipole, so that one
+// LDS address serves both
+template <bool DIPK>
+struct Staged {
+    NodeRec32 r;
+    float4 d[DIPK ? 1 : 0];
+};
+
+template <bool D, bool CNT, bool DIPK>
+__device__ inline int visit_range(const Staged<DIPK>* sr, uint32_t& rem, uint32_t lm,
                                   uint32_t hqm, uint64_t tm, float tx, float ty, float tz,
                                   uint32_t self, float soft2, float& px, float& py,
                                   float& pz, uint32_t& ncell, uint32_t& nleaf, uint32_t& nopen,
@@ -431,7 +439,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
     const bool act = (tm >> lane) & 1ull;
     for (uint32_t m = rem & lm; m; m &= m - 1u) {
         const uint32_t j = (uint32_t)__builtin_ctz(m);
-        const NodeRec32 r = sr[j];
+        const NodeRec32 r = sr[j].r;
         const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         // a target skips its own leaf (pot.c:1787); a leaf at float32
@@ -448,7 +456,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
     for (uint32_t c = rem & ~lm; c;) {
         const uint32_t j = 31u - (uint32_t)__builtin_clz(c);
         c &= ~(1u << j);
-        const NodeRec32 r = sr[j];
+        const NodeRec32 r = sr[j].r;
         const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         const bool acc = r.size2 < d2;
@@ -461,7 +469,7 @@ __device__ inline int visit_range(const NodeRec32* sr, const float4* sd, uint32_
         const float inv3 = inv2 * inv;
         if (D) {
             // DIPOLES (pot.c:1738-1745): q d / r^3 - p / r^3 + 3 (d.p) d / r^5
-            const float4 pd = sd[j];
+            const float4 pd = sr[j].d[0];
             const float rpd2 = __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
             const float ca = inv3 * __builtin_fmaf(3.f, rpd2, r.q);
             px = __builtin_fmaf(-inv3, pd.x, __builtin_fmaf(ca, dx, px));
@@ -497,8 +505,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
     __shared__ uint32_t s_first[NW][STK];
     __shared__ uint32_t s_cnt[NW][STK];  // cnt | cells still to visit << 8
     __shared__ uint64_t s_mask[NW][STK];
-    __shared__ NodeRec32 s_rec[NW][8];
-    __shared__ float4 s_dip[NW][DIP ? 8 : 1];
+    __shared__ Staged<DIP> s_rec[NW][8];
     const int wv = threadIdx.x / WAVE;
     const uint32_t lane = __lane_id();
     const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
@@ -560,8 +567,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                     if (lane < tc) mr = rec[tf + lane];
                     else mr = NodeRec32{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u, 0u};
                     if (lane < tc) {
-                        s_rec[wv][lane] = mr;
-                        if (DIP && dip) s_dip[wv][lane] = io.dip32[tf + lane];
+                        s_rec[wv][lane].r = mr;
+                        if constexpr (DIP) {
+                            if (dip) s_rec[wv][lane].d[0] = io.dip32[tf + lane];
+                        }
                     }
                     lm = (uint32_t)ballot(lane < tc && mr.cnt == LEAF);
                     // DIPOLES takes every cell holding charge, net or not
@@ -574,10 +583,10 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                 uint64_t next_mask = 0;
                 const int j =
                     (DIP && dip)
-                        ? visit_range<true, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
+                        ? visit_range<true, CNT, DIP>(s_rec[wv], rem, lm, hqm, tm, tx, ty,
                                                  tz, self, soft2, px, py, pz, ncell, nleaf,
                                                  nopen, next_first, next_cnt, next_mask)
-                        : visit_range<false, CNT>(s_rec[wv], s_dip[wv], rem, lm, hqm, tm, tx, ty,
+                        : visit_range<false, CNT, DIP>(s_rec[wv], rem, lm, hqm, tm, tx, ty,
                                                   tz, self, soft2, px, py, pz, ncell,
                                                   nleaf, nopen, next_first, next_cnt, next_mask);
                 kahan_add(fx, kx, px);
