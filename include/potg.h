@@ -159,6 +159,17 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision,
 int potg_destroy(potg_ctx* ctx);
 const char* potg_last_error(void);
 
+/* Check a particle array for instance configuration cfg as the reference's
+   build_tree does before inserting (pot.c:1503-1514, with the NaN test of
+   is_particle_position_nan, pot.c:1434-1446): particle 0 is the grain, the
+   others are ions or electrons (POTG_ERR_ARG otherwise), and every position
+   lies in the root cell [0, 2 r mpp]^3 with no NaN coordinate
+   (POTG_ERR_OUT_OF_BOX otherwise, the reference's message in
+   potg_last_error).  *bad_index (may be NULL) receives the first offending
+   index.  Host only; potg_upload calls it. */
+int potg_check_particles(const potg_config* cfg, const potg_particle* parts,
+                         uint64_t* bad_index);
+
 /* Copy one instance's particles (reference index order, grain at index 0)
    and grain state to the device, and set the step counter t (the number of
    steps already completed, pot.c:4261). */

@@ -109,9 +109,14 @@ int kernel_error(potg_ctx* c)
 int validate(const potg_config& k, uint32_t i)
 {
     if (k.N < 1 || k.N > 0x7fffffffULL) return set_err("instance %u: bad N", i), POTG_ERR_ARG;
-    if (k.r == 0 || !(k.mpp > 0) || !(k.dt > 0) || !(k.a > 0) || k.soft < 0 || !(k.Te >= 0) ||
-        !(k.Ti >= 0) || !(k.theta >= 0) || k.geometry > 2 || !(k.mime > 0))
+    if (k.r == 0 || !(k.mpp > 0) || !(k.dt > 0) || !(k.a > 0) || !(k.soft >= 0) ||
+        !(k.theta >= 0) || k.geometry > 2 || !(k.mime > 0))
         return set_err("instance %u: bad parameters", i), POTG_ERR_ARG;
+    // the ion sub-cycling factor sqrt(Te mI / (Ti mE)) (pot.c:4157-4159),
+    // the shielding length of the reinjection (pot.c:578-580) and chi_b's
+    // kT_i/e (pot.c:2278) all divide by a temperature
+    if (!(k.Te > 0) || !(k.Ti > 0) || !isfinite(k.Te) || !isfinite(k.Ti))
+        return set_err("instance %u: temperatures must be positive and finite", i), POTG_ERR_ARG;
     if (k.geometry == POTG_PROLATE && !(k.A > 1.0))
         return set_err("instance %u: prolate aspect ratio must be greater than 1", i), POTG_ERR_ARG;
     if (k.geometry == POTG_OBLATE && !(k.A < 1.0 && k.A > 0.0))
@@ -464,6 +469,36 @@ int potg_destroy(potg_ctx* c)
     return POTG_OK;
 }
 
+int potg_check_particles(const potg_config* cfg, const potg_particle* parts, uint64_t* bad_index)
+{
+    if (!cfg || !parts) return set_err("potg_check_particles: bad arguments"), POTG_ERR_ARG;
+    if (bad_index) *bad_index = 0;
+    const uint64_t N = cfg->N;
+    if (N < 1) return set_err("potg_check_particles: bad N"), POTG_ERR_ARG;
+    if (parts[0].species != POTG_DUST) return set_err("particle 0 must be the grain"), POTG_ERR_ARG;
+    const double x2 = 2.0 * cfg->r * cfg->mpp;  // the root cell's upper faces (pot.c:1476-1478)
+    for (uint64_t i = 0; i < N; i++) {
+        const potg_particle& q = parts[i];
+        if (i && q.species != POTG_ION && q.species != POTG_ELECTRON) {
+            if (bad_index) *bad_index = i;
+            return set_err("particle %llu: species must be ion or electron",
+                           (unsigned long long)i),
+                   POTG_ERR_ARG;
+        }
+        // build_tree's test (pot.c:1503-1514), written so that a NaN
+        // coordinate fails it too (is_particle_position_nan, pot.c:1434-1446)
+        if (!(q.x >= 0.0 && q.x <= x2) || !(q.y >= 0.0 && q.y <= x2) ||
+            !(q.z >= 0.0 && q.z <= x2)) {
+            if (bad_index) *bad_index = i;
+            return set_err("particle %llu at (%g, %g, %g) with velocity (%g, %g, %g) not in "
+                           "simulation area.",
+                           (unsigned long long)i, q.x, q.y, q.z, q.vx, q.vy, q.vz),
+                   POTG_ERR_OUT_OF_BOX;
+        }
+    }
+    return POTG_OK;
+}
+
 int potg_upload(potg_ctx* c, uint32_t inst, const potg_particle* parts, const potg_grain* grain,
                 uint64_t t_completed)
 {
@@ -471,13 +506,14 @@ int potg_upload(potg_ctx* c, uint32_t inst, const potg_particle* parts, const po
         return set_err("potg_upload: bad arguments"), POTG_ERR_ARG;
     const potg_config& k = c->cfg[inst];
     const size_t N = k.N;
-    if (parts[0].species != POTG_DUST) return set_err("particle 0 must be the grain"), POTG_ERR_ARG;
+    {
+        const int rc = potg_check_particles(&k, parts, nullptr);
+        if (rc) return rc;
+    }
     InstParams& p = c->hp[inst];
     for (int s = 0; s < 3; s++) p.first_idx[s] = NONE;
     for (size_t i = 1; i < N; i++) {
         const int s = parts[i].species;
-        if (s != POTG_ION && s != POTG_ELECTRON)
-            return set_err("particle %zu: species must be ion or electron", i), POTG_ERR_ARG;
         if (p.first_idx[s] == NONE) p.first_idx[s] = (uint32_t)i;
     }
     p.grain[0] = parts[0].x;

@@ -89,7 +89,10 @@ struct InstState {
     // per-step accumulators (cleared by the step's first kernel)
     int32_t acc_q;
     uint32_t acc_e, acc_i, n_pushed, n_reinj_abs, n_reinj_out, n_wall_inward;
-    double acc_p[3];
+    // absorbed momentum per axis, an exact sum in 128-bit two's-complement
+    // fixed point (units of 2^-140 kg m/s; see add_fixed128)
+    unsigned long long acc_p_lo[3];
+    unsigned long long acc_p_hi[3];
     // outputs of the last step
     int32_t out_q;
     uint32_t out_e, out_i, out_pushed, out_reinj_abs, out_reinj_out, out_wall_inward;
@@ -162,6 +165,51 @@ __device__ inline uint64_t rfl64(uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Momentum tally of absorbed particles.  Float64 atomics would add in the
+// order threads arrive, so the grain's velocity would differ between runs
+// (and between a captured step and an eager one).  Each contribution p is
+// instead scaled by 2^140 (exact), truncated to an integer and added to a
+// 128-bit two's-complement accumulator with two 64-bit integer atomics:
+// integer addition is associative, so the sum is exact and independent of
+// the order.  |p| < 2^-13 kg m/s keeps the total below 2^127; the unit,
+// 2^-140 kg m/s (about 7e-43), is far below an electron's momentum at any
+// plasma speed (about 1e-25 at 1e5 m/s).
+__device__ inline void add_fixed128(unsigned long long* lo, unsigned long long* hi, double p)
+{
+    const double x = fabs(ldexp(p, 140));
+    const double xh = floor(x * 0x1p-64);
+    // exact: x - xh 2^64 is a multiple of ulp(x) below 2^64
+    const double xl = x - xh * 0x1p64;
+    unsigned long long h = (unsigned long long)xh, l = (unsigned long long)xl;
+    if (p < 0.0) {
+        l = ~l + 1ull;
+        h = ~h + (l == 0ull ? 1ull : 0ull);
+    }
+    const unsigned long long old = atomicAdd(lo, l);
+    atomicAdd(hi, h + (old + l < old ? 1ull : 0ull));
+}
+
+// the accumulated 128-bit sum as a double (kg m/s)
+__device__ inline double fixed128_value(unsigned long long lo, unsigned long long hi)
+{
+    const bool neg = (long long)hi < 0;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    const double mag = ((double)hi * 0x1p64 + (double)lo) * 0x1p-140;
+    return neg ? -mag : mag;
+}
+
+// a kernel earlier in the step raised an error: the walk, push and
+// reinjection kernels after it skip their work rather than compute with
+// the state that raised it (NaN or out-of-box positions, coincident
+// particles)
+__device__ inline bool step_failed(const Globals* g)
+{
+    return *(volatile const int32_t*)&g->err != 0;
+}
 
 __device__ inline void raise_err(Globals* g, int code, uint32_t idx)
 {

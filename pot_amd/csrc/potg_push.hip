@@ -394,7 +394,7 @@ __global__ __launch_bounds__(BT) void k_half(const InstParams* __restrict__ P,
                                               double* __restrict__ x0)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= g->n_targets) return;
+    if (k >= g->n_targets || step_failed(g)) return;
     const uint32_t i = targets[k];
     const InstParams& p = P[st.inst[i]];
     const int s = st.spec[i];
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ntg = g->n_targets;
-    const bool valid = k < ntg;
+    const bool valid = k < ntg && !step_failed(g);
     uint32_t inst = 0;
     if (valid) {
         const uint32_t i = targets[k];
@@ -514,9 +514,9 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
                 atomicAdd(&S[inst].acc_q, s == 1 ? 1 : -1);
                 if (s == 2) atomicAdd(&S[inst].acc_e, 1u);
                 else atomicAdd(&S[inst].acc_i, 1u);
-                atomicAdd(&S[inst].acc_p[0], p.mass[s] * vx);
-                atomicAdd(&S[inst].acc_p[1], p.mass[s] * vy);
-                atomicAdd(&S[inst].acc_p[2], p.mass[s] * vz);
+                add_fixed128(&S[inst].acc_p_lo[0], &S[inst].acc_p_hi[0], p.mass[s] * vx);
+                add_fixed128(&S[inst].acc_p_lo[1], &S[inst].acc_p_hi[1], p.mass[s] * vy);
+                add_fixed128(&S[inst].acc_p_lo[2], &S[inst].acc_p_hi[2], p.mass[s] * vz);
                 atomicAdd(&S[inst].n_reinj_abs, 1u);
                 fate = 1;
             }
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
                                                   const uint32_t* __restrict__ queue,
                                                   const double* __restrict__ qprev, uint32_t qcap)
 {
-    const uint32_t n = g->n_reinj < qcap ? g->n_reinj : qcap;
+    const uint32_t n = step_failed(g) ? 0u : (g->n_reinj < qcap ? g->n_reinj : qcap);
     const uint32_t lane = __lane_id();
     const uint32_t nw = gridDim.x * (BT / WAVE);
     for (uint32_t e = blockIdx.x * (BT / WAVE) + threadIdx.x / WAVE; e < n; e += nw) {
@@ -639,9 +639,8 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
         if (p.grain_enabled) {
             // pot.c:2831-2836
             s.grain_m += (double)s.acc_i * p.mass[1] + (double)s.acc_e * p.mass[2];
-            s.grain_v[0] += s.acc_p[0] / s.grain_m;
-            s.grain_v[1] += s.acc_p[1] / s.grain_m;
-            s.grain_v[2] += s.acc_p[2] / s.grain_m;
+            for (int c = 0; c < 3; c++)
+                s.grain_v[c] += fixed128_value(s.acc_p_lo[c], s.acc_p_hi[c]) / s.grain_m;
             s.grain_q += s.acc_q;
             s.tot_e += s.acc_e;
             // the grain's own potential energy with its updated charge

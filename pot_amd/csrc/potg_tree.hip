@@ -86,9 +86,10 @@ __global__ void k_begin_step(InstParams* __restrict__ P, InstState* __restrict__
         S[k].n_reinj_abs = 0;
         S[k].n_reinj_out = 0;
         S[k].n_wall_inward = 0;
-        S[k].acc_p[0] = 0.0;
-        S[k].acc_p[1] = 0.0;
-        S[k].acc_p[2] = 0.0;
+        for (int c = 0; c < 3; c++) {
+            S[k].acc_p_lo[c] = 0ull;
+            S[k].acc_p_hi[c] = 0ull;
+        }
     }
     if (k == 0) {
         g->t = t;

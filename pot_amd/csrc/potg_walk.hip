@@ -148,7 +148,7 @@ __global__ __launch_bounds__(BT) void k_walk64(const InstParams* __restrict__ P,
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
         tile = rfl(__shfl(tile, 0, WAVE));
-        if ((uint64_t)tile * WAVE >= ntg) break;
+        if ((uint64_t)tile * WAVE >= ntg || rfl(step_failed(g) ? 1u : 0u)) break;
         const uint32_t k = tile * WAVE + lane;
         uint32_t inst, self;
         double tx, ty, tz;
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
         tile = rfl(__shfl(tile, 0, WAVE));
-        if ((uint64_t)tile * WAVE >= ntg) break;
+        if ((uint64_t)tile * WAVE >= ntg || rfl(step_failed(g) ? 1u : 0u)) break;
         const uint32_t k = tile * WAVE + lane;
         uint32_t inst, self;
         double dtx, dty, dtz;
@@ -684,7 +684,7 @@ __global__ __launch_bounds__(BT) void k_walkg(const InstParams* __restrict__ P, 
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
         tile = rfl(__shfl(tile, 0, WAVE));
-        if ((uint64_t)tile * K >= ntg) break;
+        if ((uint64_t)tile * K >= ntg || rfl(step_failed(g) ? 1u : 0u)) break;
         const uint32_t k = tile * K + t;
         uint32_t inst, self;
         double dtx, dty, dtz;
@@ -861,7 +861,7 @@ __global__ __launch_bounds__(BT) void k_walk32grp(const InstParams* __restrict__
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
         tile = rfl(__shfl(tile, 0, WAVE));
-        if ((uint64_t)tile * WAVE >= ntg) break;
+        if ((uint64_t)tile * WAVE >= ntg || rfl(step_failed(g) ? 1u : 0u)) break;
         const uint32_t k = tile * WAVE + lane;
         uint32_t inst, self;
         double dtx, dty, dtz;
@@ -1297,7 +1297,7 @@ __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, 
                                               const double* __restrict__ dip64, uint32_t n_inst,
                                               uint32_t n_rec)
 {
-    if ((g->t % 20) != 0) return;
+    if ((g->t % 20) != 0 || step_failed(g)) return;
     const uint32_t inst = blockIdx.x;
     if (inst >= n_inst || threadIdx.x != 0) return;
     const InstParams& p = P[inst];

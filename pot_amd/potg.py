@@ -28,7 +28,7 @@ EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "pot
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
            "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles", "potg_tune",
-           "potg_set_tuning", "potg_set_counting"]
+           "potg_set_tuning", "potg_set_counting", "potg_check_particles"]
 
 
 class Config(C.Structure):
@@ -104,6 +104,7 @@ def lib():
     L.potg_tree_canon_dipoles.argtypes = [P, C.c_uint32, P, C.c_int64, C.POINTER(C.c_int64)]
     L.potg_tune.argtypes = [P, C.POINTER(Tuning)]
     L.potg_set_tuning.argtypes = [P, C.POINTER(Tuning), C.POINTER(Tuning)]
+    L.potg_check_particles.argtypes = [C.POINTER(Config), P, C.POINTER(C.c_uint64)]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
@@ -141,6 +142,14 @@ def config(N, r, mpp, dt, soft, a, Te, Ti, x_drift=0.0, settle=0.0, geometry=POT
     c.out_bounce = 0 if bounce is None else 1
     c.bounce_coe = 0.0 if bounce is None else bounce
     return c
+
+
+def check_particles(cfg, parts):
+    """potg_check_particles: (code, first bad index, message); host only."""
+    parts = np.ascontiguousarray(parts, PARTICLE_DT)
+    bad = C.c_uint64()
+    rc = lib().potg_check_particles(C.byref(cfg), parts.ctypes.data, C.byref(bad))
+    return rc, bad.value, lib().potg_last_error().decode() if rc else ""
 
 
 class Context:

@@ -52,3 +52,64 @@ def gpu_ctx(case, parts, precision=G.POTG_FP64, seed=0, q=0, t=0, chi_b=0.0):
     ctx = G.Context([gpu_config(case, seed)], precision)
     ctx.upload(0, parts.view(G.PARTICLE_DT), q=q, t=t, chi_b=chi_b)
     return ctx
+
+
+def domain_radius(case):
+    return float(case["args"]["-r"]) * float(case["args"]["-m"])
+
+
+def moved_mask(case, po, p0):
+    """particles that moved more than 5% of the domain radius in the steps
+    compared: the reinjected ones"""
+    R = domain_radius(case)
+    return np.sqrt((po["x"] - p0["x"]) ** 2 + (po["y"] - p0["y"]) ** 2 +
+                   (po["z"] - p0["z"]) ** 2) > 0.05 * R
+
+
+def assert_particles_match(case, po, pg, p0, x_tol=1e-12, v_tol=1e-10):
+    """The float64 path against the oracle with the same semantics.  Bounds
+    are relative to the domain radius (positions) and the largest velocity
+    component (velocities).  Reinjected particles went through Newton
+    inversions that stop at |p| < 1e-8 (prng.c:152, 211): libm and the
+    device math library may stop one iteration apart, so those get a looser
+    bound (1e-6 R, 1e-5 of the velocity scale)."""
+    assert np.array_equal(pg["species"], po["species"])
+    R = domain_radius(case)
+    moved = moved_mask(case, po, p0)
+    vscale = np.abs(po["vx"]).max()
+    for f in ("x", "y", "z"):
+        d = np.abs(pg[f] - po[f])
+        assert d[~moved].max() <= x_tol * R, (f, d[~moved].max() / R)
+        assert d[moved].max(initial=0) <= 1e-6 * R, f
+    for f in ("vx", "vy", "vz"):
+        d = np.abs(pg[f] - po[f])
+        assert d[~moved].max() <= v_tol * vscale, (f, d[~moved].max() / vscale)
+        assert d[moved].max(initial=0) <= 1e-5 * vscale, f
+
+
+def assert_step_outputs_equal(oo, og):
+    assert og["abs_q"] == oo.abs_q
+    assert og["abs_e"] == oo.abs_e
+    assert og["n_pushed"] == oo.n_pushed
+    assert og["n_reinj_abs"] + og["n_reinj_out"] == oo.n_reinj_abs + oo.n_reinj_out
+    assert og["n_wall_inward"] == oo.n_wall_inward
+
+
+def step_both(case, steps=1, seed=77, precision=G.POTG_FP64, t0=0, chi_b=0.0, parts=None):
+    """The oracle with the GPU's semantics (frozen leaf sources, the Philox
+    reinjection stream) and a device context from the same state, the step
+    counter set to t0 steps completed and the edge potential to chi_b;
+    `steps` steps of each.  Returns (sim, ctx, oracle outputs, GPU outputs,
+    start particles)."""
+    sim = oracle_sim(case, inplace=0, rng_kind=O.RNG_PHILOX, philox_seed=seed)
+    if parts is not None:
+        sim.set_particles(parts)
+    p0 = sim.particles()
+    st = sim.state()
+    sim.set_state(st["q"], st["m"], t0, chi_b, st["tot_e"])
+    ctx = gpu_ctx(case, p0, precision=precision, seed=seed, q=st["q"], t=t0, chi_b=chi_b)
+    oo, og = [], []
+    for _ in range(steps):
+        oo.append(sim.run_step())
+        og.append(ctx.step()[0])
+    return sim, ctx, oo, og, p0

@@ -189,8 +189,10 @@ class Sim:
         t = C.c_uint64()
         chi = C.c_double()
         tot = C.c_uint32()
-        self.L.or_get_state(self.h, C.byref(q), None, C.byref(t), C.byref(chi), C.byref(tot), None)
-        return {"q": q.value, "t": t.value, "chi_b": chi.value, "tot_e": tot.value}
+        m = C.c_longdouble()
+        self.L.or_get_state(self.h, C.byref(q), C.byref(m), C.byref(t), C.byref(chi), C.byref(tot),
+                            None)
+        return {"q": q.value, "t": t.value, "chi_b": chi.value, "tot_e": tot.value, "m": m.value}
 
     def set_state(self, q, m, t, chi_b, tot_e):
         self.L.or_set_state(self.h, q, m, t, chi_b, tot_e)

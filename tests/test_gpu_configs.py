@@ -1,0 +1,242 @@
+"""The benchmark workloads that only ran in the bench, checked against the
+oracle at their own sizes:
+
+- sphere_16k (8192 electrons + 8192 ions, sphere a = 0.1 lambda_D, R = 10
+  lambda_D, no B, theta 0.5): the tree bit for bit, the float64 walk on
+  every particle with exact interaction counts, one frozen-Philox step of
+  each precision, fifty float64 steps against fifty oracle steps, and two
+  hundred captured steps that conserve charge and particles.
+- the batched sweep (64 instances of 20k-500k particles in one context):
+  the trees of four instances against the oracle's, float64 and float32
+  walk samples of those instances, batch against single-instance contexts,
+  and per-instance conservation over captured steps.
+- prolate_4m_tilted_b (4M particles, prolate grain, B 30 degrees off its
+  axis, Te/Ti = 10): the tree bit for bit and a float64 walk sample with
+  exact interaction counts.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_common import (assert_particles_match, assert_step_outputs_equal, gpu_config,
+                        gpu_ctx, oracle_sim, step_both)
+
+from pot_amd import potg as G
+from pot_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(w, name, seed):
+    return dict(w, name=name, seed=seed, steps=1)
+
+
+def _canon_equal(got, ref):
+    assert len(got["depth"]) == len(ref["depth"])
+    for f in ("depth", "is_leaf", "q", "aq", "nchild"):
+        assert np.array_equal(got[f], ref[f]), f
+    leaf = ref["is_leaf"] == 1
+    assert np.array_equal(got["part"][leaf], ref["part"][leaf])
+    assert np.array_equal(got["c"], ref["c"]), "centre of charge"
+
+
+def _oracle_walk(sim, parts, idx):
+    """the oracle's float64-accumulating walk at particles idx, with its
+    interaction counts"""
+    xyz = np.stack([parts["x"][idx], parts["y"][idx], parts["z"][idx]], 1)
+    Eo = np.zeros((len(idx), 3))
+    cnto = np.zeros((len(idx), 3), np.int64)
+    for k, i in enumerate(idx):
+        st = O.WalkStats()
+        Eo[k] = sim.field_at(int(i), xyz[k, 0], xyz[k, 1], xyz[k, 2], 1, st)
+        cnto[k] = (st.n_cell, st.n_leaf, st.n_open)
+    return xyz, Eo, cnto
+
+
+def _sample(n, k, seed):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(np.arange(1, n), size=min(k, n - 1), replace=False)).astype(np.uint32)
+
+
+SPHERE16K = _case(W.sphere_16k(), "sphere_16k", 2000)
+
+
+def test_sphere_16k_tree_and_walk_match_oracle():
+    case = SPHERE16K
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    assert len(parts) == 16385
+    sim.build_tree()
+    ctx = gpu_ctx(case, parts)
+    ctx.build_tree()
+    _canon_equal(ctx.tree_canon(0), sim.tree_canon())
+    idx = np.arange(1, len(parts), dtype=np.uint32)
+    xyz, Eo, cnto = _oracle_walk(sim, parts, idx)
+    E, cnt = ctx.field_at(0, xyz, exclude=idx, counts=True)
+    assert np.array_equal(cnt.astype(np.int64), cnto), "interaction counts differ"
+    rel = np.linalg.norm(E - Eo, axis=1) / np.linalg.norm(Eo, axis=1)
+    assert rel.max() <= 1e-13, rel.max()
+    mixed = gpu_ctx(case, parts, precision=G.POTG_MIXED)
+    mixed.build_tree()
+    rel32 = np.linalg.norm(mixed.field_at(0, xyz, exclude=idx) - Eo, axis=1) / \
+        np.linalg.norm(Eo, axis=1)
+    assert np.median(rel32) < 1e-5 and np.percentile(rel32, 99) < 2e-3
+
+
+def test_sphere_16k_step_matches_oracle():
+    case = SPHERE16K
+    sim, ctx, oo, og, p0 = step_both(case, steps=1, seed=8)
+    assert_step_outputs_equal(oo[0], og[0])
+    pg, gr = ctx.download(0)
+    assert_particles_match(case, sim.particles(), pg, p0)
+    assert gr["q"] == sim.state()["q"]
+
+
+def test_sphere_16k_fifty_steps_track_the_oracle():
+    """Fifty float64 steps (t = 1..50, chi_b measured at 20 and 40, ions
+    pushed at 42) against fifty oracle steps with the same semantics."""
+    case = SPHERE16K
+    sim, ctx, oo, og, p0 = step_both(case, steps=50, seed=9)
+    for a, b in zip(oo, og):
+        assert_step_outputs_equal(a, b)
+    pg, gr = ctx.download(0)
+    st = sim.state()
+    assert gr["q"] == st["q"] and gr["tot_e"] == st["tot_e"]
+    assert abs(gr["chi_b"] - st["chi_b"]) <= 1e-9 * abs(st["chi_b"])
+    assert_particles_match(case, sim.particles(), pg, p0, x_tol=1e-9, v_tol=1e-7)
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_sphere_16k_captured_steps_conserve(precision):
+    """Two hundred captured steps (potg_run): the grain's charge is the
+    start charge plus ions minus electrons absorbed, nothing is lost, every
+    particle stays inside the domain's sphere."""
+    case = SPHERE16K
+    parts = oracle_sim(case).particles()
+    ctx = gpu_ctx(case, parts, precision=precision, seed=4)
+    ctx.set_counting(False)
+    n_i = n_e = 0
+    for _ in range(4):
+        o = ctx.step()[0]
+        n_i += o["abs_i"]
+        n_e += o["abs_e"]
+    ctx.run(196)
+    p, g = ctx.download(0)
+    assert g["tot_e"] >= n_e
+    # ions absorbed over the 196 captured steps, from the charge balance
+    n_i_all = g["q"] + g["tot_e"]
+    assert n_i_all >= n_i
+    assert np.array_equal(np.bincount(p["species"], minlength=3),
+                          np.bincount(parts["species"], minlength=3))
+    R = float(case["args"]["-r"]) * float(case["args"]["-m"])
+    r = np.sqrt((p["x"][1:] - R) ** 2 + (p["y"][1:] - R) ** 2 + (p["z"][1:] - R) ** 2)
+    assert np.all(np.isfinite(r)) and r.max() <= R
+
+
+# ---- the batched sweep ------------------------------------------------------
+
+SWEEP = W.batched_sweep()
+CHECK = [0, 21, 42, 63]
+
+
+def _sweep_cases():
+    return [_case(w, "sweep%02d" % k, 3000 + k) for k, w in enumerate(SWEEP)]
+
+
+def _sweep_context(precision, cases, states):
+    ctx = G.Context([gpu_config(c, seed=500 + k) for k, c in enumerate(cases)], precision)
+    for k, s in enumerate(states):
+        ctx.upload(k, s)
+    return ctx
+
+
+@pytest.fixture(scope="module")
+def sweep():
+    from bench import initial_state
+    cases = _sweep_cases()
+    states = [initial_state(gpu_config(c), c["seed"]) for c in cases]
+    return cases, states
+
+
+def test_sweep_trees_and_walks_match_oracle(sweep):
+    cases, states = sweep
+    assert len(cases) == 64
+    n = [len(s) for s in states]
+    assert min(n) >= 20000 and max(n) <= 500001
+    ctx64 = _sweep_context(G.POTG_FP64, cases, states)
+    ctx64.build_tree()
+    ctx32 = _sweep_context(G.POTG_MIXED, cases, states)
+    ctx32.build_tree()
+    for k in CHECK:
+        sim = oracle_sim(cases[k])
+        parts = sim.particles()
+        assert np.array_equal(parts.view(np.uint8), states[k].view(np.uint8))
+        sim.build_tree()
+        _canon_equal(ctx64.tree_canon(k), sim.tree_canon())
+        idx = _sample(len(parts), 512, k)
+        xyz, Eo, cnto = _oracle_walk(sim, parts, idx)
+        E, cnt = ctx64.field_at(k, xyz, exclude=idx, counts=True)
+        assert np.array_equal(cnt.astype(np.int64), cnto), k
+        rel = np.linalg.norm(E - Eo, axis=1) / np.linalg.norm(Eo, axis=1)
+        assert rel.max() <= 1e-13, (k, rel.max())
+        rel32 = np.linalg.norm(ctx32.field_at(k, xyz, exclude=idx) - Eo, axis=1) / \
+            np.linalg.norm(Eo, axis=1)
+        assert np.median(rel32) < 1e-5, (k, np.median(rel32))
+
+
+def test_sweep_batch_equals_single_instances(sweep):
+    """Three steps of the 64-instance context against single-instance
+    contexts of four of its instances: the same particles, grain charge,
+    mass and velocity, bit for bit."""
+    cases, states = sweep
+    batch = _sweep_context(G.POTG_MIXED, cases, states)
+    singles = {k: gpu_ctx(cases[k], states[k], precision=G.POTG_MIXED, seed=500 + k)
+               for k in CHECK}
+    for c in [batch] + list(singles.values()):
+        c.set_tuning(2048, 64)
+    for _ in range(3):
+        batch.step()
+        for s in singles.values():
+            s.step()
+    for k, s in singles.items():
+        pb, gb = batch.download(k)
+        ps, gs = s.download(0)
+        assert np.array_equal(pb.view(np.uint8), ps.view(np.uint8)), k
+        assert gb == gs, k
+
+
+def test_sweep_captured_steps_conserve_per_instance(sweep):
+    cases, states = sweep
+    ctx = _sweep_context(G.POTG_MIXED, cases, states)
+    ctx.set_counting(False)
+    ctx.run(30)
+    for k in range(len(cases)):
+        p, g = ctx.download(k)
+        assert len(p) == len(states[k])
+        assert np.array_equal(np.bincount(p["species"], minlength=3),
+                              np.bincount(states[k]["species"], minlength=3)), k
+        assert g["q"] + g["tot_e"] >= 0, k  # ions absorbed
+        R = float(cases[k]["args"]["-r"]) * float(cases[k]["args"]["-m"])
+        r = np.sqrt((p["x"][1:] - R) ** 2 + (p["y"][1:] - R) ** 2 + (p["z"][1:] - R) ** 2)
+        assert np.all(np.isfinite(r)) and r.max() <= R, k
+
+
+# ---- the 4M prolate workload ---------------------------------------------------
+
+def test_prolate_4m_tree_and_walk_match_oracle():
+    case = _case(W.prolate_4m_tilted_b(), "prolate_4m_tilted_b", 6)
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    assert len(parts) == 4000001
+    sim.build_tree()
+    ctx = gpu_ctx(case, parts)
+    ctx.build_tree()
+    ref = sim.tree_canon()
+    _canon_equal(ctx.tree_canon(0), ref)
+    del ref
+    idx = _sample(len(parts), 512, 1)
+    xyz, Eo, cnto = _oracle_walk(sim, parts, idx)
+    E, cnt = ctx.field_at(0, xyz, exclude=idx, counts=True)
+    assert np.array_equal(cnt.astype(np.int64), cnto), "interaction counts differ"
+    rel = np.linalg.norm(E - Eo, axis=1) / np.linalg.norm(Eo, axis=1)
+    assert rel.max() <= 1e-13, rel.max()

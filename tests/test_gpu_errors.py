@@ -1,0 +1,81 @@
+"""Failure paths of the reference, forced from the host on a small case.
+Every kernel still runs to its normal end: the tree kernels bound every
+index they follow, and the walk, push and reinjection kernels of a step
+whose tree build raised an error skip their work.  After each error the
+same context steps a valid state again, which shows nothing was left
+behind on the device.
+
+- Two particles at one position: build_tree fails (pot.c:1578-1593) and
+  the walk would abort (pot.c:1800-1802); the GPU build reports
+  POTG_ERR_COINCIDENT.
+- A position outside the root cell or NaN (pot.c:1503-1514, 1434-1446):
+  potg_upload refuses it on the host; a NaN that arises on the device (a
+  NaN velocity carries the position out at the next drift) is reported by
+  the next step's tree build as POTG_ERR_OUT_OF_BOX.
+"""
+import numpy as np
+import pytest
+
+from gpu_common import big_case, gpu_ctx, oracle_sim
+
+from pot_amd import potg as G
+
+pytestmark = pytest.mark.gpu
+
+CASE = big_case(2001, seed=71)
+
+
+def _valid():
+    return oracle_sim(CASE).particles()
+
+
+def _recovers(ctx, parts):
+    ctx.upload(0, parts)
+    out = ctx.step()[0]
+    assert out["n_pushed"] > 0
+    p, _ = ctx.download(0)
+    assert np.all(np.isfinite(p["x"]))
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_coincident_particles_are_reported(precision):
+    parts = _valid()
+    bad = parts.copy()
+    for f in ("x", "y", "z"):
+        bad[f][1500] = bad[f][700]
+    ctx = gpu_ctx(CASE, bad, precision=precision)
+    with pytest.raises(G.PotgError, match="POTG_ERR_COINCIDENT"):
+        ctx.build_tree()
+    ctx.upload(0, bad)
+    with pytest.raises(G.PotgError, match="POTG_ERR_COINCIDENT"):
+        ctx.step()
+    # the step's walk and push did nothing: the particles are where they were
+    p, _ = ctx.download(0)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(p[f], bad[f]), f
+    _recovers(ctx, parts)
+
+
+def test_out_of_box_upload_is_refused():
+    parts = _valid()
+    ctx = gpu_ctx(CASE, parts)
+    R = float(CASE["args"]["-r"]) * float(CASE["args"]["-m"])
+    for v in (-1e-12, 2.0 * R * 1.000001, np.nan):
+        bad = parts.copy()
+        bad["y"][33] = v
+        with pytest.raises(G.PotgError, match="POTG_ERR_OUT_OF_BOX"):
+            ctx.upload(0, bad)
+    _recovers(ctx, parts)
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_nan_arising_on_the_device_is_reported(precision):
+    parts = _valid()
+    bad = parts.copy()
+    j = int(np.nonzero(bad["species"] == G.ELECTRON)[0][10])
+    bad["vx"][j] = np.nan  # the host check looks at positions only, as the reference does
+    ctx = gpu_ctx(CASE, bad, precision=precision)
+    ctx.step()  # the drift carries the NaN into the position
+    with pytest.raises(G.PotgError, match="POTG_ERR_OUT_OF_BOX"):
+        ctx.step()
+    _recovers(ctx, parts)

@@ -1,0 +1,139 @@
+"""Whole steps of the default mixed-precision path against the oracle.
+
+The float32 walk (k_walk32) differs from the float64 walk by float32
+rounding (median ~1e-6 of the field) and by opening decisions that flip at
+the threshold in float32 (a few targets per thousand, by up to the
+truncation error of the cells involved).  Everything else in a step -- the
+tree, the Boris push, grain crossing, tallies, reinjection -- is float64 and
+the same as the FP64 path.  So after one step each particle's velocity
+differs from the oracle's by the field difference times q/m dt:
+
+    |v_gpu - v_oracle| = (q/m) dt_s |E_gpu - E_oracle|
+
+The tests state the tolerance on r = |v_gpu - v_oracle| / ((q/m) dt_s |E|),
+the relative field error the velocities imply, with E the oracle's field
+at the particle's half-drifted position:  median r < 2e-5, 99th percentile
+< 5e-3 (the MAC flips), every particle r < 0.2.  Positions follow from the
+velocity: |x_gpu - x_oracle| <= dt_s / 2 |v_gpu - v_oracle| + 1e-12 R.
+Discrete outcomes (absorptions, reinjections, counts) must be equal.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_common import (CASES, assert_step_outputs_equal, big_case, domain_radius,
+                        moved_mask, step_both)
+
+from pot_amd import potg as G
+
+pytestmark = pytest.mark.gpu
+
+QE = 1.6021766e-19
+ME = 9.109383e-31
+
+BIG = [big_case(20001, seed=61), big_case(20001, geometry=1, A=2.0, seed=62,
+                                          B=(0.0, 0.0, 20.0)),
+       big_case(20001, geometry=2, A=0.5, seed=63),
+       dict(big_case(20001, seed=64), name="big20001_dipoles", dipoles=1)]
+BORIS = [c for c in CASES if not c.get("integrator")] + BIG
+
+
+def _field_error_ratio(case, sim0, p0, po, pg, t):
+    """r per stepped, not reinjected particle (see the module docstring)"""
+    tss = sim0.tss()
+    dt = float(case["args"]["-d"])
+    mime = 1836.153 if case["geometry"] else 1836.15
+    moved = moved_mask(case, po, p0)
+    r, dx = [], []
+    R = domain_radius(case)
+    for i in range(1, len(p0)):
+        s = int(p0["species"][i])
+        if t % tss[s] or moved[i]:
+            continue
+        dts = dt * tss[s]
+        xh = p0["x"][i] + 0.5 * dts * p0["vx"][i]
+        yh = p0["y"][i] + 0.5 * dts * p0["vy"][i]
+        zh = p0["z"][i] + 0.5 * dts * p0["vz"][i]
+        E = sim0.field_at(i, xh, yh, zh, 1)
+        qom = QE / (ME * (mime if s == 1 else 1.0))
+        dv = np.sqrt((pg["vx"][i] - po["vx"][i]) ** 2 + (pg["vy"][i] - po["vy"][i]) ** 2 +
+                     (pg["vz"][i] - po["vz"][i]) ** 2)
+        r.append(dv / (qom * dts * np.linalg.norm(E)))
+        ddx = np.sqrt((pg["x"][i] - po["x"][i]) ** 2 + (pg["y"][i] - po["y"][i]) ** 2 +
+                      (pg["z"][i] - po["z"][i]) ** 2)
+        dx.append(ddx - (0.5 * dts * dv + 1e-12 * R))
+    return np.array(r), np.array(dx)
+
+
+@pytest.mark.parametrize("case", BORIS, ids=[c["name"] for c in BORIS])
+def test_mixed_step_matches_oracle(case):
+    sim, ctx, oo, og, p0 = step_both(case, steps=1, precision=G.POTG_MIXED, seed=31)
+    assert_step_outputs_equal(oo[0], og[0])
+    po = sim.particles()
+    pg, gr = ctx.download(0)
+    assert np.array_equal(pg["species"], po["species"])
+    assert gr["q"] == sim.state()["q"]
+    # the oracle's own field at the half-drifted positions, from the same
+    # start-of-step tree
+    sim0 = _start_tree(case, p0)
+    r, dx = _field_error_ratio(case, sim0, p0, po, pg, 1)
+    assert len(r) > 0
+    assert np.median(r) < 2e-5, np.median(r)
+    assert np.percentile(r, 99) < 5e-3, np.percentile(r, 99)
+    assert r.max() < 0.2, r.max()
+    assert dx.max() <= 0.0, dx.max()
+    # reinjected particles: same Philox draws, float64 sampling
+    moved = moved_mask(case, po, p0)
+    R = domain_radius(case)
+    for f in ("x", "y", "z"):
+        assert np.abs(pg[f] - po[f])[moved].max(initial=0) <= 1e-6 * R, f
+
+
+def _start_tree(case, p0):
+    from gpu_common import oracle_sim
+    s = oracle_sim(case, inplace=0)
+    s.set_particles(p0)
+    s.build_tree()
+    return s
+
+
+@pytest.mark.parametrize("case", [BIG[0], BIG[1], BIG[3]], ids=["sphere", "prolate", "dipoles"])
+def test_mixed_steps_track_the_oracle(case):
+    """Five steps: the float32 field errors do not compound beyond the
+    float32 walk's own error (velocities relative to the field's kick),
+    the tallies stay equal, and the grain charge stays equal."""
+    sim, ctx, oo, og, p0 = step_both(case, steps=5, precision=G.POTG_MIXED, seed=33)
+    for a, b in zip(oo, og):
+        assert_step_outputs_equal(a, b)
+    po = sim.particles()
+    pg, gr = ctx.download(0)
+    assert gr["q"] == sim.state()["q"]
+    moved = moved_mask(case, po, p0)
+    vscale = np.abs(po["vx"]).max()
+    dv = np.abs(pg["vx"] - po["vx"])[~moved]
+    # five kicks of E dt: the velocity scale stays far above the difference
+    assert np.median(dv) < 1e-6 * vscale, np.median(dv) / vscale
+    assert np.percentile(dv, 99) < 5e-4 * vscale, np.percentile(dv, 99) / vscale
+
+
+def test_mixed_program_matches_oracle(tmp_path):
+    """The drop-in program on its default (mixed) path: 12 macro records
+    against the oracle.  Columns that integrate the field (kinetic
+    energies, momenta, speeds) agree to 1e-5 relative; the potential
+    energies, computed by the float64 walk from positions that moved by
+    dt/2 times the velocity difference, agree to 1e-5; the grain charge
+    and the absorbed-electron count are exact."""
+    import struct
+    from make_golden import splitmix64_words
+    from test_gpu_host import case_args, close_lines, oracle_macro_lines, run_pot
+    case = big_case(2001, seed=5)
+    seed = 4242
+    (tmp_path / "seed.bin").write_bytes(struct.pack("<16I", *splitmix64_words(seed)))
+    steps = 12
+    out = run_pot(tmp_path, case_args(case) + ["-i", str(steps), "--seed-file",
+                                               str(tmp_path / "seed.bin")], "run")
+    got = open(out / "pot-macroscopic.dat").read().splitlines(True)
+    assert len(got) == steps + 1
+    want = oracle_macro_lines(case, seed, steps)
+    for t in range(steps):
+        close_lines(got[t + 1], want[t], rel=1e-5)

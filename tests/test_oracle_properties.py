@@ -132,3 +132,81 @@ def test_frozen_and_live_sources_agree_before_anything_moves():
     for i in range(1, len(p), 29):
         x, y, z = p["x"][i], p["y"][i], p["z"][i]
         assert np.array_equal(a.field_at(i, x, y, z, 0), b.field_at(i, x, y, z, 0))
+
+
+# ---- the SCEPTIC reinjection's distributions (pot.c:496-652) -------------------
+#
+# Without drift and at chi = 0, the speed at infinity u (in units of
+# v_n = sqrt(2 kT/m)) follows Hutchinson's (2003) eq. 19 at U = 0,
+#     F(u) = 1 - (u^2 + 1 - chi) exp(-u^2) / (1 - chi)
+# (prng.c:138-154, pot.c:546-562); the impact parameter b has b^2 uniform
+# on [0, 1 - chi/u^2] (pot.c:582); energy and angular momentum conservation
+# give the boundary velocity (pot.c:617-633), so |v|^2/v_n^2 = u^2 - chi
+# and (v.r_hat)^2 / |v|^2 is uniform on [0, 1]; and with no preferred
+# direction the position is uniform on the sphere.  At chi != 0 a sample
+# whose orbit meets a potential barrier (alpha NaN) is drawn again
+# (pot.c:584-599), which reweights u and b by a factor with no closed form;
+# there the position must still be isotropic and every particle must have
+# u^2 >= max(chi, 0).  A dropped term in the energy balance, a wrong sign
+# of chi or a transposed rotation breaks one of these.
+
+def no_flow_cdf(u, chi):
+    return 1.0 - (u * u + 1.0 - chi) * np.exp(-u * u) / (1.0 - chi)
+
+
+def conditional_speed_cdf(u, chi):
+    """F conditioned on u^2 >= chi"""
+    lo = no_flow_cdf(np.sqrt(max(chi, 0.0)), chi) if chi > 0 else 0.0
+    return (no_flow_cdf(u, chi) - lo) / (1.0 - lo)
+
+
+def reinjection_invariants(parts, R, v_n, chi_s):
+    """(u-CDF values, (v.r_hat)^2/|v|^2, cos polar angle, azimuth/2pi) of
+    reinjected particles at positions (x, y, z) on the sphere of radius R
+    about (R, R, R)"""
+    d = np.stack([parts["x"] - R, parts["y"] - R, parts["z"] - R], 1)
+    r = np.linalg.norm(d, axis=1)
+    v = np.stack([parts["vx"], parts["vy"], parts["vz"]], 1)
+    v2 = np.sum(v * v, axis=1)
+    u = np.sqrt(v2 / (v_n * v_n) + chi_s)
+    cos_vr2 = (np.sum(v * d, axis=1) / r) ** 2 / v2
+    return (conditional_speed_cdf(u, chi_s), cos_vr2, d[:, 2] / r,
+            (np.arctan2(d[:, 1], d[:, 0]) / (2 * np.pi)) % 1.0, np.sum(v * d, axis=1))
+
+
+@pytest.mark.parametrize("chi_b", [0.0, 0.3, -0.6])
+@pytest.mark.parametrize("species", [O.ION, O.ELECTRON], ids=["ion", "electron"])
+def test_oracle_reinjection_distributions(chi_b, species):
+    from scipy import stats
+    case = CASES[0]  # sphere, Te = Ti, no drift
+    sim = oracle_sim(case)
+    L = sim.L
+    rng = np.random.default_rng(17 + species)
+    n = 6000
+    out = np.zeros(n, O.PARTICLE_DT)
+    p = np.zeros(1, O.PARTICLE_DT)
+    for k in range(n):
+        p[0]["species"] = species
+        u = np.ascontiguousarray(rng.random(1024))
+        used = L.or_sceptic_from_uniforms(sim.h, p.ctypes.data, chi_b, O.dptr(u), 1024)
+        assert used > 0
+        out[k] = p[0]
+    R = float(case["args"]["-r"]) * float(case["args"]["-m"])
+    T = float(case["args"]["-E"] if species == O.ELECTRON else case["args"]["-I"])
+    m = 9.109383e-31 * (1.0 if species == O.ELECTRON else 1836.15)
+    v_n = np.sqrt(2 * 1.38065e-23 * T / m)
+    # electrons see chi_b scaled by -Ti/Te (pot.c:536-539)
+    chi_s = chi_b if species == O.ION else -chi_b * float(case["args"]["-I"]) / float(
+        case["args"]["-E"])
+    Fu, cos_vr2, cz, az, vdotr = reinjection_invariants(out, R, v_n, chi_s)
+    r = np.sqrt((out["x"] - R) ** 2 + (out["y"] - R) ** 2 + (out["z"] - R) ** 2)
+    assert np.allclose(r, R, rtol=1e-12)
+    assert np.all(vdotr < 0)  # every particle enters the domain
+    v2 = out["vx"] ** 2 + out["vy"] ** 2 + out["vz"] ** 2
+    assert np.all(v2 / v_n ** 2 + chi_s >= max(chi_s, 0.0) * (1 - 1e-12))
+    checks = [("cos polar", (cz + 1) / 2), ("azimuth", az)]
+    if chi_b == 0.0:
+        checks += [("speed", Fu), ("radial fraction", cos_vr2)]
+    for name, x in checks:
+        pval = stats.kstest(x, "uniform").pvalue
+        assert pval > 1e-3, (name, pval)
