@@ -36,6 +36,9 @@ extern "C" {
 #define POTG_ERR_COINCIDENT (-4) /* two particles at one position (pot.c:1578-1586, 1799-1802) */
 #define POTG_ERR_DEPTH (-5)      /* tree deeper than the walk stack (pot.c:1647-1652) */
 #define POTG_ERR_NOMEM (-6)      /* device allocation failed */
+#define POTG_ERR_REINJECT (-7)   /* SCEPTIC reinjection found no admissible entry in
+                                    1000 tries: chi_b bars the species (the reference
+                                    retries without end, pot.c:583-598) */
 
 /* species codes, pot.c:286-290 */
 #define POTG_DUST 0
@@ -138,6 +141,9 @@ typedef struct potg_stats_t {
     uint64_t n_particles;
     uint64_t max_depth;
     uint64_t n_range_loads; /* child ranges fetched by walk wavefronts */
+    uint64_t n_child_visits; /* children visited by float32-walk wavefronts (64
+                                targets each): (n_cell + n_leaf) / (64 n_child_visits)
+                                is the fraction of lanes doing an interaction */
 } potg_stats_t;
 
 /* Launch parameters of the field walk (potg_tune picks them by timing). */
@@ -146,7 +152,10 @@ typedef struct potg_tuning {
     uint32_t group;       /* float32 walk: targets per wavefront, 8, 16 or 64;
                              with 8 or 16 the wavefront evaluates 8 or 4
                              children per target at once */
-    uint32_t pad[2];
+    uint32_t variant;     /* float32 walk with 64 targets: 0 visits a range's
+                             cells until some lanes open one, 1 sweeps every
+                             range whole before descending */
+    uint32_t sort_tile;   /* keys per tile of the radix sort: 2048 or 4096 */
     double walk_ms;       /* potg_tune: the chosen walk's time (ms) */
 } potg_tuning;
 

@@ -51,17 +51,20 @@ build_pot() {  # name theta bx by bz [extra -D...]; NO_BORIS=1: velocity Verlet
             -DRANDOMNESS_SOURCE="${SEED_SOURCE:-\"seed.bin\"}" "$@" -o "$OUT/$name" -lm
 }
 
-build_sph() {  # name shape theta bx by bz; NO_BORIS=1: velocity Verlet
+build_sph() {  # name shape theta bx by bz [extra -D...]; NO_BORIS=1: velocity Verlet;
+              # NO_MACRO=1: no pot-macroscopic.dat (RECORD_MACRO_DATA, SPH:68)
     local name=$1 shape=$2 theta=$3 bx=$4 by=$5 bz=$6
+    shift 6
     # the shipped file defines PROLATE (SPH:113); OBLATE replaces it
     local extra=() shape_def=()
     if [ "$shape" = OBLATE ]; then extra=(-e '/^#define PROLATE$/d'); shape_def=(-DOBLATE); fi
     if [ -n "${NO_BORIS:-}" ]; then extra+=(-e '/^#define BORIS$/d'); fi
+    if [ -n "${NO_MACRO:-}" ]; then extra+=(-e '/^#define RECORD_MACRO_DATA /d'); fi
     tr -d '\r' < "$REF/pot-spheroid.c" | sed "${SPH_DEL[@]}" "${extra[@]}" |
         MPICH_CC=gcc "$MPICC" -x c - "$REF/prng.c" -I"$REF" -std=c99 -Os \
             "${shape_def[@]}" -DOAP="$theta" \
             -DEXTERNAL_B_X="$bx" -DEXTERNAL_B_Y="$by" -DEXTERNAL_B_Z="$bz" \
-            -DRANDOMNESS_SOURCE='"seed.bin"' -o "$OUT/$name" -lm
+            -DRANDOMNESS_SOURCE="${SEED_SOURCE:-\"seed.bin\"}" "$@" -o "$OUT/$name" -lm
 }
 
 # spherical grain, SCEPTIC reinjection, Boris
@@ -89,5 +92,13 @@ build_sph pot_prolate PROLATE 0.5 43.30127018922193 0.0 25.0
 build_sph pot_oblate OBLATE 0.5 0.0 0.0 0.0
 build_sph pot_prolate_4m PROLATE 0.5 1.61238 0.0 0.93091
 NO_BORIS=1 build_sph pot_prolate_verlet PROLATE 0.5 0.0 0.0 0.0
+# prolate_4m_tilted_b for timing under mpiexec (bench.py's CPU leg): ranks
+# seed from /dev/urandom, and the macroscopic record -- which on every one of
+# the first 200 steps adds a potential walk over all particles (pot.c:2771-2795,
+# 4364) that the GPU's timed steps do not take -- is compiled out.  One variant
+# runs the reference's own algorithm (monopoles, OAP 0.5), the other the
+# setting of the headline (DIPOLES, OAP 0.14).
+SEED_SOURCE='"/dev/urandom"' NO_MACRO=1 build_sph pot_prolate_4m_mpi PROLATE 0.5 1.61238 0.0 0.93091
+SEED_SOURCE='"/dev/urandom"' NO_MACRO=1 build_sph pot_prolate_4m_dip_mpi PROLATE 0.14 1.61238 0.0 0.93091 -DDIPOLES
 
 echo "build_ref: built $(ls "$OUT" | wc -l) reference variants in $OUT"

@@ -100,6 +100,10 @@ int kernel_error(potg_ctx* c)
     case KERR_CAPACITY:
         set_err("particle tree larger than its preallocated capacity (%u)", idx);
         return POTG_ERR_NOMEM;
+    case KERR_REINJECT:
+        set_err("particle (global slot %u) found no admissible SCEPTIC reinjection in 1000 "
+                "tries: chi_b bars its species from entering", idx);
+        return POTG_ERR_REINJECT;
     default:
         set_err("particle tree path length exceeded (node %u)", idx);
         return POTG_ERR_DEPTH;
@@ -328,6 +332,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_tree = (uint32_t)n_tree;
     b.n_inst = n_inst;
     b.precision = precision;
+    b.sort_ipt = 16;
     for (uint32_t i = 0; i < n_inst; i++) b.any_spheroid |= configs[i].geometry != POTG_SPHERE;
     b.count = 1;
     int ib = 0;
@@ -345,6 +350,7 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
         const bool large = tiles >= 16u * (uint64_t)device_cus();
         b.walk_blocks = (large ? 8u : 4u) * (uint32_t)device_cus();
         b.group = (large || precision == POTG_FP64) ? 64u : 16u;  // float64: always 64
+        b.variant = 0;
     }
 
     c->hp.resize(n_inst);
@@ -402,11 +408,14 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
         rc |= alloc_n(c, &b.dip64, 3 * (size_t)b.n_rec);
         if (precision != POTG_FP64) rc |= alloc_n(c, &b.dip32, b.n_rec);
     }
-    rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, nt);
+    const size_t tc = nt + (size_t)WAVE * n_inst;
+    b.t_cap = (uint32_t)tc;
+    rc |= alloc_n(c, &b.flags, nt) | alloc_n(c, &b.tpos, nt) | alloc_n(c, &b.targets, tc);
+    rc |= alloc_n(c, &b.t_first, n_inst) | alloc_n(c, &b.t_base, n_inst);
     rc |= alloc_n(c, &b.queue, nt);
-    rc |= alloc_n(c, &b.E, 3 * nt) | alloc_n(c, &b.inv, n);
+    rc |= alloc_n(c, &b.E, 3 * tc) | alloc_n(c, &b.inv, n);
     b.integrator = configs[0].integrator;
-    if (b.integrator != POTG_BORIS) rc |= alloc_n(c, &b.a1, 3 * nt) | alloc_n(c, &b.x0, 3 * nt);
+    if (b.integrator != POTG_BORIS) rc |= alloc_n(c, &b.a1, 3 * tc) | alloc_n(c, &b.x0, 3 * tc);
     bool any_bounce = false;
     for (uint32_t i = 0; i < n_inst; i++) any_bounce |= configs[i].out_bounce != 0;
     if (any_bounce) rc |= alloc_n(c, &b.qprev, 3 * nt);
@@ -871,6 +880,7 @@ int potg_stats(potg_ctx* c, potg_stats_t* out)
     out->n_particles = c->b.n_total;
     out->max_depth = g.max_depth;
     out->n_range_loads = g.n_pops;
+    out->n_child_visits = g.n_visits;
     return POTG_OK;
 }
 
@@ -887,7 +897,7 @@ int potg_stats_reset(potg_ctx* c)
     if (!c) return set_err("potg_stats_reset: null context"), POTG_ERR_ARG;
     Globals g;
     HIPCHK(hipMemcpy(&g, c->b.g, sizeof(g), hipMemcpyDeviceToHost));
-    g.n_cell = g.n_leaf = g.n_open = g.n_tgt_total = g.n_steps = g.n_pops = 0;
+    g.n_cell = g.n_leaf = g.n_open = g.n_tgt_total = g.n_steps = g.n_pops = g.n_visits = 0;
     g.max_depth = 0;
     HIPCHK(hipMemcpy(c->b.g, &g, sizeof(g), hipMemcpyHostToDevice));
     return POTG_OK;
@@ -898,17 +908,22 @@ int potg_set_tuning(potg_ctx* c, const potg_tuning* set, potg_tuning* get)
     if (!c) return set_err("potg_set_tuning: null context"), POTG_ERR_ARG;
     if (set) {
         if (set->walk_blocks == 0 || set->walk_blocks > 65536 ||
-            (set->group != 8 && set->group != 16 && set->group != 64))
-            return set_err("potg_set_tuning: walk_blocks must be 1..65536, group 8, 16 or 64"),
+            (set->group != 8 && set->group != 16 && set->group != 64) || set->variant > 1 ||
+            (set->sort_tile != 0 && set->sort_tile != 2048 && set->sort_tile != 4096))
+            return set_err("potg_set_tuning: walk_blocks must be 1..65536, group 8, 16 or 64, "
+                           "variant 0 or 1, sort_tile 2048 or 4096 (0: unchanged)"),
                    POTG_ERR_ARG;
         c->b.walk_blocks = set->walk_blocks;
         c->b.group = set->group;
+        c->b.variant = set->variant;
+        if (set->sort_tile) c->b.sort_ipt = set->sort_tile / 256;
         destroy_graphs(c);  // captured steps hold the old launch parameters
     }
     if (get) {
         get->walk_blocks = c->b.walk_blocks;
         get->group = c->b.group;
-        get->pad[0] = get->pad[1] = 0;
+        get->variant = c->b.variant;
+        get->sort_tile = 256 * c->b.sort_ipt;
         get->walk_ms = c->tuned_ms;
     }
     return POTG_OK;
@@ -933,19 +948,22 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     rc = kernel_error(c);
     if (rc) return rc;
     const uint32_t cus = (uint32_t)device_cus();
-    // candidates: targets per wavefront (float32 walk only) x blocks per CU
-    const uint32_t groups[] = {8, 16, 64};
+    // candidates: targets per wavefront and visit order (float32 walk only:
+    // 8, 16, or 64 in either order) x blocks per CU
+    const uint32_t shapes[][2] = {{8, 0}, {16, 0}, {64, 0}, {64, 1}};
     const uint32_t bpc[] = {4, 8, 16};
-    const uint32_t keep_wb = b.walk_blocks, keep_gr = b.group;
+    const uint32_t keep_wb = b.walk_blocks, keep_gr = b.group, keep_var = b.variant;
     double best = 1e30;
-    uint32_t best_wb = keep_wb, best_gr = keep_gr;
+    uint32_t best_wb = keep_wb, best_gr = keep_gr, best_var = keep_var;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     for (uint32_t wb : bpc) {
-        for (uint32_t gr : groups) {
+        for (const auto& sh : shapes) {
+            const uint32_t gr = sh[0];
             // the float64 and group-criterion walks have one shape: only the
             // grid varies
-            if (b.precision != POTG_MIXED && gr != 64) continue;
+            if (b.precision != POTG_MIXED && (gr != 64 || sh[1] != 0)) continue;
             b.group = gr;
+            b.variant = sh[1];
             b.walk_blocks = wb * cus;
             float ms_best = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
@@ -963,9 +981,37 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
                 best = ms_best;
                 best_wb = b.walk_blocks;
                 best_gr = b.group;
+                best_var = b.variant;
             }
         }
     }
+    // the radix sort's tile: the keys, sort and permute of the coming step
+    // again under each tile size (the state is sorted already, so the order
+    // stays as it is)
+    const uint32_t keep_ipt = b.sort_ipt;
+    uint32_t best_ipt = keep_ipt;
+    if (!rc) {
+        float sbest = 1e30f;
+        for (uint32_t ipt : {8u, 16u}) {
+            b.sort_ipt = ipt;
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(e0, s));
+                launch_keys_sort_permute(b, s);
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < ms_best) ms_best = ms;
+            }
+            if (ms_best < sbest) {
+                sbest = ms_best;
+                best_ipt = ipt;
+            }
+        }
+        rc = kernel_error(c);
+    }
+    b.sort_ipt = rc ? keep_ipt : best_ipt;
     rc = kernel_error(c);
     // the step was not taken: step counter and work counters as before
     Globals g1;
@@ -975,11 +1021,13 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     g1.n_leaf = g0.n_leaf;
     g1.n_open = g0.n_open;
     g1.n_pops = g0.n_pops;
+    g1.n_visits = g0.n_visits;
     g1.n_tgt_total = g0.n_tgt_total;
     g1.n_steps = g0.n_steps;
     HIPCHK(hipMemcpy(b.g, &g1, sizeof(g1), hipMemcpyHostToDevice));
     b.walk_blocks = rc ? keep_wb : best_wb;
     b.group = rc ? keep_gr : best_gr;
+    b.variant = rc ? keep_var : best_var;
     c->tuned_ms = rc ? 0.0 : best;
     destroy_graphs(c);
     if (rc) return rc;

@@ -36,6 +36,7 @@ constexpr int KERR_OUT_OF_BOX = 1;
 constexpr int KERR_COINCIDENT = 2;
 constexpr int KERR_DEPTH = 3;
 constexpr int KERR_CAPACITY = 4;
+constexpr int KERR_REINJECT = 5;
 
 // Constant parameters of one instance (one grain simulation).
 struct InstParams {
@@ -101,18 +102,20 @@ struct InstState {
 // Context-wide device counters.
 struct Globals {
     uint64_t t;            // steps completed
-    uint32_t n_targets;    // walk targets of the current step
+    uint32_t n_targets;    // target slots of the current step: each instance's
+                           // targets start at a multiple of 64, the gaps hold NONE
     uint32_t n_internal;   // internal tree nodes
     uint32_t n_children;   // sum of child counts
     uint32_t tile_ctr;     // persistent-walk work counter
     uint32_t tile_ctr2;
     uint32_t n_reinj;      // reinjection queue length of the current step
-    uint32_t pad_;
+    uint32_t n_targets_real; // particles stepped this step (targets without the gaps)
     int32_t err;
     uint32_t err_idx;
     uint32_t max_depth;
     unsigned long long n_cell, n_leaf, n_open, n_tgt_total, n_steps;
     unsigned long long n_pops;   // child ranges fetched by the walk (per wavefront)
+    unsigned long long n_visits; // children visited by float32-walk wavefronts
 };
 
 // Tree node records, children contiguous: an internal node's children are

@@ -26,6 +26,8 @@ struct Bufs {
     int key_bits;      // bits of the key that are sorted
     uint32_t walk_blocks; // blocks of the persistent walk grid
     uint32_t group;    // float32 walk: targets per wavefront (8, 16 or 64)
+    uint32_t variant;  // with 64: 0 visits cells until one opens, 1 sweeps whole ranges
+    uint32_t sort_ipt; // radix-sort elements per thread: tiles of 256 x (8 or 16)
     InstParams* P;
     InstState* S;
     Globals* g;
@@ -55,10 +57,15 @@ struct Bufs {
     float4* dip32;     // the same in units of R for the float32 walk
     uint32_t n_rec;    // record capacity
     uint32_t cap_int;  // internal-node capacity
-    // targets of the walk and push
+    // targets of the walk and push: per instance a run starting at a
+    // multiple of 64 (a wavefront's tile never mixes instances), so t_cap =
+    // n_tree + 64 n_inst slots; t_first / t_base: each instance's first
+    // target in the unpadded count and its run's start
     uint32_t* flags;
     uint32_t* tpos;
     uint32_t* targets;
+    uint32_t t_cap;
+    uint32_t *t_first, *t_base;
     uint32_t* queue;   // reinjection queue
     double* E;
     // velocity Verlet / Euler-Richardson: the first half's acceleration and

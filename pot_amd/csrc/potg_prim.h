@@ -17,8 +17,9 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_
 size_t radix_temp_words(uint32_t n);
 // Stable LSD sort of (keys, vals) on bits [0, end_bit); the sorted data end
 // up in keys/vals (the pointers are swapped with the _alt buffers as needed).
+// ipt: elements per thread of a tile, 8 or 16 (tiles of 2048 or 4096).
 void radix_sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt,
-                      uint32_t n, int end_bit, uint32_t* temp, hipStream_t s);
+                      uint32_t n, int end_bit, uint32_t* temp, int ipt, hipStream_t s);
 
 void iota(uint32_t* v, uint32_t n, hipStream_t s);
 // out[k] = i for the k-th i with flags[i] != 0, in order; *count = total

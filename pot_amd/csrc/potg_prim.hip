@@ -140,24 +140,25 @@ __global__ __launch_bounds__(BT) void k_scan_down(const uint32_t* __restrict__ i
 
 // ---- radix sort -------------------------------------------------------
 //
-// 8-bit digits, three launches per pass: the digit counts of every
-// 4096-element tile (k_rs_count), a scan of each digit's counts over the
+// 8-bit digits, three launches per pass: the digit counts of every tile of
+// 256 x RS_IPT elements (k_rs_count), a scan of each digit's counts over the
 // tiles (k_rs_scan, one block per digit), and k_rs_scatter, which ranks its
 // tile stably by digit with wave ballots, stages it in LDS in digit order
 // and then stores every digit's run to its global place, so that
-// neighbouring lanes store to neighbouring addresses.
+// neighbouring lanes store to neighbouring addresses.  The tile is 2048 or
+// 4096 elements (RS_IPT 8 or 16, picked by potg_tune): a smaller tile gives
+// more blocks, and so fills the device, when the keys are few.
 
-constexpr int RS_IPT = 16;                     // elements per thread
-constexpr int RS_TILE = BT * RS_IPT;           // elements per block
 constexpr int RS_WAVES = BT / WAVE;
-constexpr int RS_WTILE = RS_TILE / RS_WAVES;   // contiguous elements per wavefront
 
-inline uint32_t rs_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+inline uint32_t rs_blocks(uint32_t n, int ipt) { return (n + BT * ipt - 1) / (BT * ipt); }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(BT) void k_rs_count(const uint64_t* __restrict__ keys, uint32_t n,
                                                   int shift, uint32_t nb,
                                                   uint32_t* __restrict__ cnt)
 {
+    constexpr int RS_TILE = BT * RS_IPT;
     __shared__ uint32_t h[RS_WAVES][256];
 #pragma unroll
     for (int k = 0; k < RS_WAVES; k++) h[k][threadIdx.x] = 0;
@@ -209,6 +210,7 @@ __global__ __launch_bounds__(BT) void k_rs_scan(uint32_t* __restrict__ cnt, uint
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(BT) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                     const uint32_t* __restrict__ vin,
                                                     uint64_t* __restrict__ kout,
@@ -217,6 +219,8 @@ __global__ __launch_bounds__(BT) void k_rs_scatter(const uint64_t* __restrict__ 
                                                     const uint32_t* __restrict__ cnt,
                                                     const uint32_t* __restrict__ tot)
 {
+    constexpr int RS_TILE = BT * RS_IPT;
+    constexpr int RS_WTILE = RS_TILE / RS_WAVES;  // contiguous elements per wavefront
     __shared__ uint64_t skey[RS_TILE];
     __shared__ uint32_t sval[RS_TILE];
     __shared__ uint32_t wc[RS_WAVES][256];  // running digit counts per wavefront
@@ -330,19 +334,26 @@ void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_
     k_scan_down<<<nb, BT, 0, s>>>(in, n, n_dev, temp, out, add);
 }
 
-size_t radix_temp_words(uint32_t n) { return 256ull * (rs_blocks(n) + 1) + 64; }
+// sized for the smaller tile, which needs more per-tile counts
+size_t radix_temp_words(uint32_t n) { return 256ull * (rs_blocks(n, 8) + 1) + 64; }
 
 void radix_sort_pairs(uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals, uint32_t*& vals_alt,
-                      uint32_t n, int end_bit, uint32_t* temp, hipStream_t s)
+                      uint32_t n, int end_bit, uint32_t* temp, int ipt, hipStream_t s)
 {
     if (n == 0) return;
-    const uint32_t nb = rs_blocks(n);
+    ipt = ipt == 8 ? 8 : 16;
+    const uint32_t nb = rs_blocks(n, ipt);
     uint32_t* cnt = temp;
     uint32_t* tot = temp + 256ull * nb;
     for (int shift = 0; shift < end_bit; shift += 8) {
-        k_rs_count<<<nb, BT, 0, s>>>(keys, n, shift, nb, cnt);
+        if (ipt == 8) k_rs_count<8><<<nb, BT, 0, s>>>(keys, n, shift, nb, cnt);
+        else k_rs_count<16><<<nb, BT, 0, s>>>(keys, n, shift, nb, cnt);
         k_rs_scan<<<256, BT, 0, s>>>(cnt, nb, tot);
-        k_rs_scatter<<<nb, BT, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, nb, cnt, tot);
+        if (ipt == 8)
+            k_rs_scatter<8><<<nb, BT, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, nb, cnt, tot);
+        else
+            k_rs_scatter<16><<<nb, BT, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, nb, cnt,
+                                                tot);
         uint64_t* tk = keys; keys = keys_alt; keys_alt = tk;
         uint32_t* tv = vals; vals = vals_alt; vals_alt = tv;
     }

@@ -221,9 +221,13 @@ __device__ inline void vel_cart(const double* v, double phi, double theta, doubl
     vz = (v[0] * cos(theta)) - (v[2] * sin(theta));
 }
 
-// sceptically_reinject, pot.c:496-652 (the reference's NaN recursion is a
-// bounded retry with chi_b restored exactly as pot.c:591-596 does)
-__device__ void sceptic_reinject(const InstParams& p, int species, double chi_b_in, Stream& rs,
+// sceptically_reinject, pot.c:496-652.  The reference retries by recursion
+// while alpha comes out NaN (pot.c:583-598), restoring chi_b as it does; here
+// the retry is bounded.  When every try fails -- a chi_b that bars the
+// species from entering, e.g. after a particle came within the softening
+// length of chi_b's sample point, where the reference recurses without end --
+// the particle is left as it was and false returned.
+__device__ bool sceptic_reinject(const InstParams& p, int species, double chi_b_in, Stream& rs,
                                  double& x, double& y, double& z, double& vx, double& vy, double& vz)
 {
     double a[2], alpha = 0.0, b = 0.0, c, co_z, phi, psi, q, U, u_mag = 1.0, u2 = 1.0, v_n = 0.0;
@@ -271,6 +275,7 @@ __device__ void sceptic_reinject(const InstParams& p, int species, double chi_b_
         if (!isnan(alpha)) break;
         if (species == 2) chi_b /= -p.Ti / p.Te;
     }
+    if (isnan(alpha)) return false;
     phi = rs.draw(2.0 * PI);
     x = p.R * sin(alpha) * cos(phi);
     y = p.R * sin(alpha) * sin(phi);
@@ -289,6 +294,7 @@ __device__ void sceptic_reinject(const InstParams& p, int species, double chi_b_
     vx *= v_n;
     vy *= v_n;
     vz *= v_n;
+    return true;
 }
 
 // plain spherical reinjection at the boundary (inject, pot.c:710-765)
@@ -396,6 +402,7 @@ __global__ __launch_bounds__(BT) void k_half(const InstParams* __restrict__ P,
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g->n_targets || step_failed(g)) return;
     const uint32_t i = targets[k];
+    if (i == NONE) return;  // the gap before the next instance's targets
     const InstParams& p = P[st.inst[i]];
     const int s = st.spec[i];
     const double dts = p.dt * (double)p.tss[s];
@@ -437,7 +444,7 @@ __global__ __launch_bounds__(BT) void k_push(const InstParams* __restrict__ P, I
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ntg = g->n_targets;
-    const bool valid = k < ntg && !step_failed(g);
+    const bool valid = k < ntg && !step_failed(g) && targets[k] != NONE;
     uint32_t inst = 0;
     if (valid) {
         const uint32_t i = targets[k];
@@ -579,7 +586,8 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
         const double chi_b = S[inst].chi_b;
         double x = st.x[i], y = st.y[i], z = st.z[i];
         double vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
-        if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
+        bool ok = true;
+        if (p.sceptic) ok = sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
         else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
         if ((code & 1u) && p.out_bounce) {
             // the absorbed particle's start position is where a bounce
@@ -591,11 +599,15 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
         } else if (code & 1u) {
             const double rad = p.R;
             const double dx = x - rad, dy = y - rad, dz = z - rad;
-            if (sqrt(dx * dx + dy * dy + dz * dz) > rad) {
+            if (ok && sqrt(dx * dx + dy * dy + dz * dz) > rad) {
                 if (lane == 0) atomicAdd(&S[inst].n_reinj_out, 1u);
-                if (p.sceptic) sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
+                if (p.sceptic) ok = sceptic_reinject(p, s, chi_b, rs, x, y, z, vx, vy, vz);
                 else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
             }
+        }
+        if (!ok) {
+            if (lane == 0) raise_err(g, KERR_REINJECT, st.ref[i]);
+            continue;  // the particle keeps its state; the step reports the error
         }
         if (lane == 0) {
             st.x[i] = x;
@@ -616,6 +628,65 @@ __global__ void k_target_flags(const InstParams* __restrict__ P, const InstState
     const uint32_t inst = st.inst[i];
     const uint32_t local = st.ref[i] - P[inst].offset;
     flags[i] = (st.spec[i] != 0 && local >= S[inst].start) ? 1u : 0u;
+}
+
+// The step's targets grouped by instance, each instance's run starting at a
+// multiple of 64: a wavefront's 64 targets then never span two instances,
+// and an instance's tiles are the same whether it runs alone or in a batch,
+// so its float32 sums -- whose grouping follows the tile's traversal -- are
+// too.  One block: the padded run sizes are scanned over the instances,
+// and each instance's gap is filled with NONE.
+__global__ __launch_bounds__(1024) void k_target_bases(const InstParams* __restrict__ P,
+                                                       uint32_t n_inst, uint32_t n_tree,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ tpos,
+                                                       uint32_t* __restrict__ first,
+                                                       uint32_t* __restrict__ base,
+                                                       uint32_t* __restrict__ targets, Globals* g)
+{
+    __shared__ uint32_t sh[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t total = n_tree ? tpos[n_tree - 1] + flags[n_tree - 1] : 0u;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < n_inst; c0 += 1024) {
+        const uint32_t j = c0 + t;
+        uint32_t cnt = 0, f = 0;
+        if (j < n_inst) {
+            // the instance's tree slots [tree_begin, tree_begin + tree_n)
+            const uint32_t b0 = P[j].tree_begin, e0 = b0 + P[j].tree_n;
+            f = b0 < n_tree ? tpos[b0] : total;
+            cnt = (e0 < n_tree ? tpos[e0] : total) - f;
+        }
+        const uint32_t sz = (cnt + WAVE - 1) / WAVE * WAVE;
+        sh[t] = sz;
+        __syncthreads();
+        for (uint32_t d = 1; d < 1024; d <<= 1) {
+            const uint32_t v = t >= d ? sh[t - d] : 0u;
+            __syncthreads();
+            sh[t] += v;
+            __syncthreads();
+        }
+        const uint32_t start = carry + sh[t] - sz;
+        if (j < n_inst) {
+            first[j] = f;
+            base[j] = start;
+            for (uint32_t q = start + cnt; q < start + sz; q++) targets[q] = NONE;
+        }
+        carry += sh[1023];
+        __syncthreads();
+    }
+    if (t == 0) g->n_targets = carry;
+}
+
+__global__ void k_scatter_targets(PState st, uint32_t n_tree, const uint32_t* __restrict__ flags,
+                                  const uint32_t* __restrict__ tpos,
+                                  const uint32_t* __restrict__ first,
+                                  const uint32_t* __restrict__ base, uint32_t* __restrict__ targets)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_tree || !flags[i]) return;
+    const uint32_t inst = st.inst[i];
+    targets[base[inst] + (tpos[i] - first[inst])] = i;
 }
 
 __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals* g, uint32_t n_inst,
@@ -659,7 +730,7 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
         s.out_wall_inward = s.n_wall_inward;
     }
     if (k == 0) {
-        g->n_tgt_total += g->n_targets;
+        g->n_tgt_total += g->n_targets_real;
         g->n_steps += 1;
     }
 }
@@ -817,19 +888,24 @@ __global__ __launch_bounds__(BT) void k_direct(const InstParams* __restrict__ P,
 void launch_targets(Bufs& b, hipStream_t s)
 {
     k_target_flags<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.st, b.n_tree, b.flags);
-    compact_flags(b.flags, b.tpos, b.n_tree, b.targets, &b.g->n_targets, b.scan_tmp, s);
+    exclusive_scan(b.flags, b.tpos, b.n_tree, nullptr, &b.g->n_targets_real, nullptr, b.scan_tmp,
+                   0, s);
+    k_target_bases<<<1, 1024, 0, s>>>(b.P, b.n_inst, b.n_tree, b.flags, b.tpos, b.t_first,
+                                      b.t_base, b.targets, b.g);
+    k_scatter_targets<<<grid(b.n_tree), BT, 0, s>>>(b.st, b.n_tree, b.flags, b.tpos, b.t_first,
+                                                    b.t_base, b.targets);
 }
 
 void launch_push(Bufs& b, hipStream_t s)
 {
-    k_push<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.a1, b.x0, b.queue,
-                                         b.qprev, b.n_tree);
+    k_push<<<grid(b.t_cap), BT, 0, s>>>(b.P, b.S, b.g, b.st, b.targets, b.E, b.a1, b.x0, b.queue,
+                                        b.qprev, b.n_tree);
     k_reinject<<<256, BT, 0, s>>>(b.P, b.S, b.g, b.st, b.queue, b.qprev, b.n_tree);
 }
 
 void launch_half(Bufs& b, hipStream_t s)
 {
-    k_half<<<grid(b.n_tree), BT, 0, s>>>(b.P, b.g, b.st, b.targets, b.E, b.a1, b.x0);
+    k_half<<<grid(b.t_cap), BT, 0, s>>>(b.P, b.g, b.st, b.targets, b.E, b.a1, b.x0);
 }
 
 void launch_finish_step(Bufs& b, int compute_epe, hipStream_t s)

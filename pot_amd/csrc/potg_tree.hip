@@ -707,7 +707,8 @@ void launch_leaf_refresh(Bufs& b, hipStream_t s)
 void launch_keys_sort_permute(Bufs& b, hipStream_t s)
 {
     k_keys<<<grid(b.n_total), BT, 0, s>>>(b.P, b.st, b.n_total, b.key_levels, b.key, b.perm, b.g);
-    radix_sort_pairs(b.key, b.key_alt, b.perm, b.perm_alt, b.n_total, b.key_bits, b.sort_tmp, s);
+    radix_sort_pairs(b.key, b.key_alt, b.perm, b.perm_alt, b.n_total, b.key_bits, b.sort_tmp,
+                     (int)b.sort_ipt, s);
     k_tiefix<<<grid(b.n_tree), BT, 0, s>>>(b.key, b.perm, b.P, b.st, b.n_tree, b.g);
     k_permute<<<grid(b.n_total), BT, 0, s>>>(b.perm, b.st, b.alt, b.n_total, b.P, b.S);
     PState t = b.st;

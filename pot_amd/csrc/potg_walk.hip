@@ -96,6 +96,7 @@ __device__ inline bool load_target(const InstParams* __restrict__ P, PState st, 
     if (k >= ntg) return false;
     if (MODE == MODE_STEP) {
         const uint32_t i = io.targets[k];
+        if (i == NONE) return false;  // the gap before the next instance's targets
         self = i;
         inst = st.inst[i];
         if (P[inst].integrator == 0) {
@@ -513,6 +514,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
     // for the context's totals when counting
     constexpr bool CNT = MODE == MODE_POINTS || COUNT;
     unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
+    unsigned long long tot_vis = 0;  // children visited by the wavefront (counting only)
 
     for (;;) {
         uint32_t tile = 0;
@@ -581,6 +583,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                 float px = 0.f, py = 0.f, pz = 0.f;  // this range visit's float32 partial
                 uint32_t next_first = 0, next_cnt = 0;
                 uint64_t next_mask = 0;
+                const uint32_t rem_in = rem;
                 const int j =
                     (DIP && dip)
                         ? visit_range<true, CNT, DIP>(s_rec[wv], rem, lm, hqm, tm, tx, ty,
@@ -592,6 +595,7 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
                 kahan_add(fx, kx, px);
                 kahan_add(fy, ky, py);
                 kahan_add(fz, kz, pz);
+                if (COUNT) tot_vis += (uint32_t)(__builtin_popcount(rem_in) - __builtin_popcount(rem));
                 if (j < 0) continue;
                 wopen += (uint32_t)__popcll(next_mask);
                 if (next_first <= tf + (uint32_t)j || next_cnt - 1u > 7u ||
@@ -640,6 +644,243 @@ __global__ __launch_bounds__(BT) void k_walk32(const InstParams* __restrict__ P,
             atomicAdd(&g->n_leaf, tot_leaf);
             atomicAdd(&g->n_open, tot_open);
             atomicAdd(&g->n_pops, tot_pops);
+            atomicAdd(&g->n_visits, tot_vis);
+        }
+    }
+}
+
+// ---- float32 walk, whole-range sweeps -------------------------------------
+//
+// The traversal of k_walk32 with another visit order.  A freshly fetched
+// range is swept whole: its leaves, then every one of its cells, youngest
+// first, each lane applying its own MAC and the accepting lanes adding the
+// cell; the cells that some lanes open are only noted.  Then the wavefront
+// descends into the youngest opened cell with the lanes that opened it; the
+// others wait on the stack as (range, opened cells, lanes).  When such an
+// entry is popped, the next opened cell's record is read alone (a uniform
+// load) and its opening lanes found again by the same MAC -- nothing is
+// added, the range's interactions were all taken by the sweep.  The sweep
+// has no exit in the middle, so each child costs its arithmetic and a few
+// scalar mask operations; every target still gets exactly the reference's
+// interaction set (the order of the sums differs from k_walk32's).
+template <bool D, bool CNT, bool DIPK>
+__device__ inline uint32_t sweep_range(const Staged<DIPK>* sr, uint32_t tc, uint32_t lm,
+                                       uint32_t hqm, uint64_t tm, float tx, float ty, float tz,
+                                       uint32_t self, float soft2, float& px, float& py, float& pz,
+                                       uint32_t& ncell, uint32_t& nleaf, uint32_t& nopen,
+                                       uint32_t& wopen, uint64_t& first_om)
+{
+    const uint32_t lane = __lane_id();
+    const bool act = (tm >> lane) & 1ull;
+    const uint32_t all = (1u << tc) - 1u;
+    for (uint32_t m = all & lm; m; m &= m - 1u) {
+        const uint32_t j = (uint32_t)__builtin_ctz(m);
+        const NodeRec32 r = sr[j].r;
+        const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const bool pn = r.part != self, dn = d2 != 0.f;
+        const float inv = __builtin_amdgcn_rsqf(d2 + soft2);
+        const bool use = act && pn && dn;
+        const float ca = use ? r.q * (inv * inv * inv) : 0.f;
+        px = __builtin_fmaf(ca, dx, px);
+        py = __builtin_fmaf(ca, dy, py);
+        pz = __builtin_fmaf(ca, dz, pz);
+        if (CNT) nleaf += use ? 1u : 0u;
+    }
+    uint32_t opened = 0;
+    first_om = 0;
+    for (uint32_t c = all & ~lm; c;) {
+        const uint32_t j = 31u - (uint32_t)__builtin_clz(c);
+        c &= ~(1u << j);
+        const NodeRec32 r = sr[j].r;
+        const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const bool acc = r.size2 < d2;
+        const uint64_t am = ballot(acc);
+        const bool use = act && acc;
+        const float inv = use ? __builtin_amdgcn_rsqf(d2) : 0.f;
+        const float inv2 = inv * inv;
+        const float inv3 = inv2 * inv;
+        if (D) {
+            const float4 pd = sr[j].d[0];
+            const float rpd2 = __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
+            const float ca = inv3 * __builtin_fmaf(3.f, rpd2, r.q);
+            px = __builtin_fmaf(-inv3, pd.x, __builtin_fmaf(ca, dx, px));
+            py = __builtin_fmaf(-inv3, pd.y, __builtin_fmaf(ca, dy, py));
+            pz = __builtin_fmaf(-inv3, pd.z, __builtin_fmaf(ca, dz, pz));
+        } else {
+            const float ca = r.q * inv3;
+            px = __builtin_fmaf(ca, dx, px);
+            py = __builtin_fmaf(ca, dy, py);
+            pz = __builtin_fmaf(ca, dz, pz);
+        }
+        if (CNT && ((hqm >> j) & 1u)) ncell += use ? 1u : 0u;
+        const uint64_t om = tm & ~am;
+        if (om) {
+            if (CNT && ((om >> lane) & 1ull)) nopen++;
+            wopen += (uint32_t)__popcll(om);
+            if (!opened) first_om = om;  // the youngest opened cell, descended first
+            opened |= 1u << j;
+        }
+    }
+    return opened;
+}
+
+template <int MODE, bool DIP, bool COUNT>
+__global__ __launch_bounds__(BT) void k_walk32s(const InstParams* __restrict__ P, Globals* g,
+                                                 PState st, const NodeRec32* __restrict__ rec,
+                                                 WalkIO io)
+{
+    __shared__ uint32_t s_first[NW][STK];
+    __shared__ uint32_t s_cnt[NW][STK];  // cnt | opened cells still to descend << 8
+    __shared__ uint64_t s_mask[NW][STK];
+    __shared__ Staged<DIP> s_rec[NW][8];
+    const int wv = threadIdx.x / WAVE;
+    const uint32_t lane = __lane_id();
+    const uint32_t ntg = (MODE == MODE_STEP) ? *io.n_targets_dev : io.n_pts;
+    constexpr bool CNT = MODE == MODE_POINTS || COUNT;
+    unsigned long long tot_cell = 0, tot_leaf = 0, tot_open = 0, tot_pops = 0;
+    unsigned long long tot_vis = 0;
+
+    for (;;) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(io.tile_ctr, 1u);
+        tile = rfl(__shfl(tile, 0, WAVE));
+        if ((uint64_t)tile * WAVE >= ntg || rfl(step_failed(g) ? 1u : 0u)) break;
+        const uint32_t k = tile * WAVE + lane;
+        uint32_t inst, self;
+        double dtx, dty, dtz;
+        const bool valid = load_target<MODE>(P, st, io, k, ntg, inst, self, dtx, dty, dtz);
+        const InstParams& p = P[inst];
+        const float tx = (float)((dtx - p.R) * p.invR);
+        const float ty = (float)((dty - p.R) * p.invR);
+        const float tz = (float)((dtz - p.R) * p.invR);
+        const float soft2 = (float)(p.soft2 * p.invR * p.invR);
+        float fx = 0.f, fy = 0.f, fz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        uint32_t ncell = 0, nleaf = 0, nopen = 0;
+        uint32_t wopen = 0;
+
+        uint64_t pending = ballot(valid);
+        while (pending) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t gi = rfl(__shfl(inst, lead, WAVE));
+            const uint64_t grp = rfl64(ballot(valid && inst == gi));
+            pending &= ~grp;
+            const bool dip = DIP && P[gi].dipoles && io.dip32;
+            if (rfl(rec[gi].cnt) == 0u) continue;
+            // the range to sweep next: the root's
+            uint32_t tf = gi, tc = 1;
+            uint64_t tm = grp;
+            int sp = 0;
+            uint32_t loads = 0;
+            for (;;) {
+                if (++loads > io.n_rec || tc - 1u > 7u) {
+                    raise_err(g, tc > 8 ? KERR_CAPACITY : KERR_DEPTH, tf);
+                    break;
+                }
+                tot_pops++;
+                uint32_t lm = 0, hqm = 0;
+                {
+                    NodeRec32 mr;
+                    if (lane < tc) mr = rec[tf + lane];
+                    else mr = NodeRec32{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u, 0u};
+                    if (lane < tc) {
+                        s_rec[wv][lane].r = mr;
+                        if constexpr (DIP) {
+                            if (dip) s_rec[wv][lane].d[0] = io.dip32[tf + lane];
+                        }
+                    }
+                    lm = (uint32_t)ballot(lane < tc && mr.cnt == LEAF);
+                    hqm = (uint32_t)ballot(lane < tc && ((DIP && dip) ? mr.part != 0u : mr.q != 0.f));
+                }
+                __builtin_amdgcn_wave_barrier();
+                float px = 0.f, py = 0.f, pz = 0.f;
+                uint64_t om = 0;
+                uint32_t opened =
+                    (DIP && dip)
+                        ? sweep_range<true, CNT, DIP>(s_rec[wv], tc, lm, hqm, tm, tx, ty, tz, self,
+                                                      soft2, px, py, pz, ncell, nleaf, nopen,
+                                                      wopen, om)
+                        : sweep_range<false, CNT, DIP>(s_rec[wv], tc, lm, hqm, tm, tx, ty, tz,
+                                                       self, soft2, px, py, pz, ncell, nleaf,
+                                                       nopen, wopen, om);
+                kahan_add(fx, kx, px);
+                kahan_add(fy, ky, py);
+                kahan_add(fz, kz, pz);
+                if (COUNT) tot_vis += tc;
+                uint32_t j = 0, next_first = 0, next_cnt = 0;
+                if (opened) {
+                    j = 31u - (uint32_t)__builtin_clz(opened);
+                    opened &= ~(1u << j);
+                    next_first = rfl(s_rec[wv][j].r.first);
+                    next_cnt = rfl(s_rec[wv][j].r.cnt);
+                } else {
+                    // the next opened cell of a range waiting on the stack:
+                    // its record alone, and the lanes that open it, found
+                    // again by the sweep's own arithmetic
+                    while (sp > 0 && om == 0) {
+                        const uint32_t cr = rfl(s_cnt[wv][sp - 1]);
+                        uint32_t rest = cr >> 8;
+                        tf = rfl(s_first[wv][sp - 1]);
+                        tm = rfl64(s_mask[wv][sp - 1]);
+                        j = 31u - (uint32_t)__builtin_clz(rest);
+                        rest &= ~(1u << j);
+                        if (rest) s_cnt[wv][sp - 1] = (cr & 0xffu) | (rest << 8);
+                        else --sp;
+                        const NodeRec32 r = rec[tf + j];
+                        const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
+                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        om = tm & ~ballot(r.size2 < d2);
+                        next_first = rfl(r.first);
+                        next_cnt = rfl(r.cnt);
+                    }
+                    if (om == 0) break;  // stack empty: the instance group is done
+                }
+                if (next_first <= tf + j || next_cnt - 1u > 7u || next_first + next_cnt > io.n_rec) {
+                    raise_err(g, KERR_CAPACITY, tf + j);
+                    break;
+                }
+                if (opened) {
+                    if (sp >= STK) {
+                        raise_err(g, KERR_DEPTH, tf);
+                        break;
+                    }
+                    s_first[wv][sp] = tf;
+                    s_cnt[wv][sp] = tc | (opened << 8);
+                    s_mask[wv][sp] = tm;
+                    sp++;
+                }
+                tf = next_first;
+                tc = next_cnt;
+                tm = om;
+            }
+        }
+        if (valid) {
+            const double K = QE / (K4PE0 * p.R * p.R);
+            io.out[3 * k] = (io.acc ? io.out[3 * k] : 0.0) + K * ((double)fx - (double)kx);
+            io.out[3 * k + 1] = (io.acc ? io.out[3 * k + 1] : 0.0) + K * ((double)fy - (double)ky);
+            io.out[3 * k + 2] = (io.acc ? io.out[3 * k + 2] : 0.0) + K * ((double)fz - (double)kz);
+            if (COUNT && io.counts) {
+                io.counts[3 * k] = ncell;
+                io.counts[3 * k + 1] = nleaf;
+                io.counts[3 * k + 2] = nopen;
+            }
+        }
+        tot_cell += ncell;
+        tot_leaf += nleaf;
+        if (lane == 0) tot_open += wopen;
+    }
+    if (COUNT) {
+        for (int d = 1; d < WAVE; d <<= 1) {
+            tot_cell += __shfl_xor(tot_cell, d, WAVE);
+            tot_leaf += __shfl_xor(tot_leaf, d, WAVE);
+        }
+        if (lane == 0) {
+            atomicAdd(&g->n_cell, tot_cell);
+            atomicAdd(&g->n_leaf, tot_leaf);
+            atomicAdd(&g->n_open, tot_open);
+            atomicAdd(&g->n_pops, tot_pops);
+            atomicAdd(&g->n_visits, tot_vis);
         }
     }
 }
@@ -1145,7 +1386,7 @@ void launch_walk_init(Bufs& b, hipStream_t s)
     // only the spheroids' analytic grain has a term to start from
     if (!b.any_spheroid) return;
     WalkIO io = step_io(b);
-    const uint32_t nt = b.n_tree ? b.n_tree : 1;
+    const uint32_t nt = b.t_cap ? b.t_cap : 1;
     k_grain_init<MODE_STEP, KIND_FIELD><<<(nt + BT - 1) / BT, BT, 0, s>>>(b.P, b.S, b.st, io);
 }
 
@@ -1162,6 +1403,9 @@ static void launch_walk32c(Bufs& b, int blocks, const WalkIO& io, hipStream_t s)
     if (b.precision == POTG_MIXED_GROUP) {
         if (dip) POTG_W32((k_walk32grp<MODE, true, COUNT>));
         else POTG_W32((k_walk32grp<MODE, false, COUNT>));
+    } else if (b.group == 64 && b.variant == 1) {
+        if (dip) POTG_W32((k_walk32s<MODE, true, COUNT>));
+        else POTG_W32((k_walk32s<MODE, false, COUNT>));
     } else if (b.group == 8) {
         if (dip) POTG_W32((k_walkg<8, MODE, true, COUNT>));
         else POTG_W32((k_walkg<8, MODE, false, COUNT>));

@@ -15,13 +15,15 @@ LIB_PATH = os.path.join(HERE, "_build", "libpotg.so")
 POTG_SPHERE, POTG_PROLATE, POTG_OBLATE = 0, 1, 2
 POTG_FP64, POTG_MIXED, POTG_MIXED_GROUP = 0, 1, 2
 POTG_BORIS, POTG_VERLET, POTG_EULER_RICHARDSON = 0, 1, 2
+POTG_ERR_ARG = -1
 DUST, ION, ELECTRON = 0, 1, 2
 
 PARTICLE_DT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),
                         ("vy", "<f8"), ("vz", "<f8"), ("species", "<i4"), ("pad", "<i4")])
 
 ERRORS = {-1: "POTG_ERR_ARG", -2: "POTG_ERR_HIP", -3: "POTG_ERR_OUT_OF_BOX",
-          -4: "POTG_ERR_COINCIDENT", -5: "POTG_ERR_DEPTH", -6: "POTG_ERR_NOMEM"}
+          -4: "POTG_ERR_COINCIDENT", -5: "POTG_ERR_DEPTH", -6: "POTG_ERR_NOMEM",
+          -7: "POTG_ERR_REINJECT"}
 
 # every entry point include/potg.h declares
 EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "potg_download",
@@ -57,12 +59,12 @@ class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("n_targets", C.c_uint64), ("n_cell", C.c_uint64),
                 ("n_leaf", C.c_uint64), ("n_open", C.c_uint64), ("n_nodes", C.c_uint64),
                 ("n_particles", C.c_uint64), ("max_depth", C.c_uint64),
-                ("n_range_loads", C.c_uint64)]
+                ("n_range_loads", C.c_uint64), ("n_child_visits", C.c_uint64)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("walk_blocks", C.c_uint32), ("group", C.c_uint32),
-                ("pad", C.c_uint32 * 2), ("walk_ms", C.c_double)]
+                ("variant", C.c_uint32), ("sort_tile", C.c_uint32), ("walk_ms", C.c_double)]
 
 
 class PotgError(RuntimeError):
@@ -259,16 +261,18 @@ class Context:
     def tune(self):
         t = Tuning()
         _check(self.L.potg_tune(self.h, C.byref(t)))
-        return {"walk_blocks": t.walk_blocks, "group": t.group, "walk_ms": t.walk_ms}
+        return {"walk_blocks": t.walk_blocks, "group": t.group, "variant": t.variant,
+                "sort_tile": t.sort_tile, "walk_ms": t.walk_ms}
 
     def tuning(self):
         t = Tuning()
         _check(self.L.potg_set_tuning(self.h, None, C.byref(t)))
-        return {"walk_blocks": t.walk_blocks, "group": t.group, "walk_ms": t.walk_ms}
+        return {"walk_blocks": t.walk_blocks, "group": t.group, "variant": t.variant,
+                "sort_tile": t.sort_tile, "walk_ms": t.walk_ms}
 
-    def set_tuning(self, walk_blocks, group=64):
+    def set_tuning(self, walk_blocks, group=64, variant=0, sort_tile=0):
         t = Tuning()
-        t.walk_blocks, t.group = walk_blocks, group
+        t.walk_blocks, t.group, t.variant, t.sort_tile = walk_blocks, group, variant, sort_tile
         _check(self.L.potg_set_tuning(self.h, C.byref(t), None))
 
     def stats(self):

@@ -9,7 +9,8 @@ The PRNG seed is the 64-byte file ./seed.bin (RANDOMNESS_SOURCE, pot.c:192)
 expanded from an integer by splitmix64, so a test can rebuild it.
 
 Only this container has the reference; the fixtures travel, the reference
-does not.  Usage: python tests/golden/make_golden.py
+does not.  Usage: python tests/golden/make_golden.py [--ensemble]
+(--ensemble: only the 2000-step runs of the CPU-runnable workload, below)
 """
 import json
 import math
@@ -107,7 +108,7 @@ def cases():
     return cs
 
 
-def run_case(c, steps, workdir):
+def run_case(c, steps, workdir, timeout=600):
     os.makedirs(os.path.join(workdir, "output"), exist_ok=True)
     with open(os.path.join(workdir, "seed.bin"), "wb") as f:
         f.write(seed_bytes(c["seed"]))
@@ -116,13 +117,82 @@ def run_case(c, steps, workdir):
         argv += [k, v]
     argv += ["-i", str(steps), "-f", "state.bin"]
     subprocess.run(argv, cwd=workdir, check=True, stdout=subprocess.DEVNULL,
-                   stderr=subprocess.DEVNULL, timeout=600)
+                   stderr=subprocess.DEVNULL, timeout=timeout)
     return workdir
+
+
+# Ensemble of the CPU-runnable workload (8192 e + 8192 i, a = 0.1 lambda_D,
+# R = 10 lambda_D, no B, no flow, theta 0.5, dt = 0.01/omega_pe; the
+# sphere_16k workload of pot_amd/workloads.py), 2000 steps per seed.  The
+# reference records every step up to t = 200 and then every 2000th
+# (pot.c:4364-4368 with settle 0); the columns kept are TIME, DGQ, TE, TI
+# and TOTE of its pot-macroscopic.dat (pot.c:3247-3510).
+ENSEMBLE_SEEDS = tuple(range(1001, 1011))
+ENSEMBLE_STEPS = 2000
+ENSEMBLE_TIMEOUT = 900  # s; the finished runs take about 6 minutes
+ENSEMBLE_COLS = ("TIME", "DGQ", "TE", "TI", "TOTE")
+
+
+def ensemble_args():
+    sys.path.insert(0, REPO)
+    from pot_amd import workloads as W
+    return dict(W.sphere_16k()["args"])
+
+
+def run_ensemble_seed(seed, dest):
+    """True when the reference finished the run.  A run can fail to finish:
+    when chi_b bars electrons from entering, sceptically_reinject recurses
+    without end (pot.c:583-598)."""
+    out = os.path.join(dest, "seed_%d.dat" % seed)
+    if os.path.exists(out):
+        return True
+    c = {"bin": "pot_sphere", "args": ensemble_args(), "seed": seed}
+    wd = tempfile.mkdtemp(prefix="potens_")
+    try:
+        try:
+            run_case(c, ENSEMBLE_STEPS, wd, timeout=ENSEMBLE_TIMEOUT)
+        except subprocess.TimeoutExpired:
+            return False
+        lines = open(os.path.join(wd, "output", "pot-macroscopic.dat")).read().split("\n")
+        head = lines[0].split()
+        cols = [head.index(k) for k in ENSEMBLE_COLS]
+        rows = [" ".join(ENSEMBLE_COLS)]
+        for ln in lines[1:]:
+            f = ln.split()
+            if f:
+                rows.append(" ".join(f[k] for k in cols))
+        with open(out, "w") as fh:
+            fh.write("\n".join(rows) + "\n")
+        return True
+    finally:
+        shutil.rmtree(wd)
+
+
+def ensemble():
+    """The reference's runs of the CPU-runnable workload, one per seed, in
+    parallel (one process each); runs already written are kept."""
+    from concurrent.futures import ThreadPoolExecutor
+    dest = os.path.join(HERE, "ensemble_16k")
+    os.makedirs(dest, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(ENSEMBLE_SEEDS))) as ex:
+        done = list(ex.map(lambda s: run_ensemble_seed(s, dest), ENSEMBLE_SEEDS))
+    with open(os.path.join(dest, "ensemble.json"), "w") as fh:
+        json.dump({"args": ensemble_args(), "bin": "pot_sphere", "theta": 0.5,
+                   "steps": ENSEMBLE_STEPS, "columns": list(ENSEMBLE_COLS),
+                   "seeds": [s for s, d in zip(ENSEMBLE_SEEDS, done) if d],
+                   "not_finished": {str(s): "no end within %d s: sceptically_reinject kept "
+                                            "retrying (pot.c:583-598)" % ENSEMBLE_TIMEOUT
+                                    for s, d in zip(ENSEMBLE_SEEDS, done) if not d}},
+                  fh, indent=1)
+    print("wrote", dest)
 
 
 def main():
     if not os.path.isdir(REF_BIN):
         sys.exit("build the reference variants first: oracle/build_ref.sh")
+    if len(sys.argv) > 1 and sys.argv[1] == "--ensemble":
+        ensemble()
+        return
     all_cases = cases()
     for c in all_cases:
         dest = os.path.join(HERE, c["name"])

@@ -130,7 +130,9 @@ def test_sphere_16k_captured_steps_conserve(precision):
                           np.bincount(parts["species"], minlength=3))
     R = float(case["args"]["-r"]) * float(case["args"]["-m"])
     r = np.sqrt((p["x"][1:] - R) ** 2 + (p["y"][1:] - R) ** 2 + (p["z"][1:] - R) ** 2)
-    assert np.all(np.isfinite(r)) and r.max() <= R
+    # reinjection puts particles on the sphere of radius R itself, where the
+    # radius recomputed here can round an ulp or two above R
+    assert np.all(np.isfinite(r)) and r.max() <= R * (1.0 + 1e-12)
 
 
 # ---- the batched sweep ------------------------------------------------------
@@ -206,10 +208,18 @@ def test_sweep_batch_equals_single_instances(sweep):
 
 
 def test_sweep_captured_steps_conserve_per_instance(sweep):
+    """Nineteen captured steps of all 64 instances: per instance nothing is
+    lost and every particle stays in its domain.  The run stops before the
+    first chi_b measurement (t = 20): there instance 30 (Te/Ti = 10, Mach
+    0.35) samples chi_b = -15.3 at its edge point, the reference's u_icdf
+    (prng.c:187-211) then returns negative or NaN speeds, and a reinjected
+    ion leaves the domain within a step -- the CPU oracle, stepping the
+    instance with the GPU's semantics, does the same at step 21, and the
+    reference's build_tree would stop the run there (pot.c:1503-1514)."""
     cases, states = sweep
     ctx = _sweep_context(G.POTG_MIXED, cases, states)
     ctx.set_counting(False)
-    ctx.run(30)
+    ctx.run(19)
     for k in range(len(cases)):
         p, g = ctx.download(k)
         assert len(p) == len(states[k])
@@ -218,7 +228,7 @@ def test_sweep_captured_steps_conserve_per_instance(sweep):
         assert g["q"] + g["tot_e"] >= 0, k  # ions absorbed
         R = float(cases[k]["args"]["-r"]) * float(cases[k]["args"]["-m"])
         r = np.sqrt((p["x"][1:] - R) ** 2 + (p["y"][1:] - R) ** 2 + (p["z"][1:] - R) ** 2)
-        assert np.all(np.isfinite(r)) and r.max() <= R, k
+        assert np.all(np.isfinite(r)) and r.max() <= R * (1.0 + 1e-12), k  # as above
 
 
 # ---- the 4M prolate workload ---------------------------------------------------

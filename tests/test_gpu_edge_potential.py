@@ -137,9 +137,15 @@ def test_reinjection_at_nonzero_chi_b(case, chi_b):
     """One step at a fixed chi_b != 0 (not a measuring step): every
     reinjection samples u with u^2 >= chi_b (scaled by -Ti/Te for electrons,
     pot.c:546-560), integrates alpha with chi_b (pot.c:433-492) and sets the
-    velocity from energy conservation (pot.c:617-633).  Four steps, so that
-    enough particles reinject."""
-    sim, ctx, oo, og, p0 = step_both(case, steps=4, t0=0, chi_b=chi_b, seed=91)
+    velocity from energy conservation (pot.c:617-633).  Ten steps of the
+    small cases (t = 1..10, before the first measuring step), so that every
+    case reinjects some particles: the small prolate case first reinjects at
+    its fifth step.  Four of the 20001-particle ones: over more steps one
+    of their many reinjections can come out grazing the boundary, where the
+    last bit of a float64 sine decides whether the next push leaves it
+    inside, and GPU and host libraries round such bits differently."""
+    steps = 4 if len(case["args"]) and int(case["args"]["-N"]) > 10000 else 10
+    sim, ctx, oo, og, p0 = step_both(case, steps=steps, t0=0, chi_b=chi_b, seed=91)
     n_re = 0
     for a, b in zip(oo, og):
         assert_step_outputs_equal(a, b)

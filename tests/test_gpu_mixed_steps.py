@@ -116,16 +116,41 @@ def test_mixed_steps_track_the_oracle(case):
     assert np.percentile(dv, 99) < 5e-4 * vscale, np.percentile(dv, 99) / vscale
 
 
+def _close_mixed(a, b, rel, mom_atol):
+    """macro records: exact integers in DGQ and TOTE; relative agreement
+    elsewhere, except the total linear and angular momenta of each species
+    (LMEX..AMIZ), sums over ~1000 particles that cancel to 1e-3 of their
+    terms, held to rel times the scale of a term sum (mom_atol[k]) rather
+    than to their own small value, and the skewness and excess kurtosis of
+    the speeds, held to an absolute 1e-4."""
+    fa, fb = a.split(), b.split()
+    assert len(fa) == len(fb) == 35
+    for k, (x, y) in enumerate(zip(fa, fb)):
+        if k in (9, 34):
+            assert x == y, (k, x, y)
+            continue
+        vx, vy = float(x), float(y)
+        tol = rel * max(abs(vx), abs(vy)) + 1e-300
+        if 22 <= k <= 33:
+            tol = max(tol, mom_atol[k])
+        if k in (14, 15, 18, 19):
+            # skewness and excess kurtosis of the speeds (pot.c:3251-3257):
+            # O(1) statistics near zero, held to an absolute 1e-4
+            tol = max(tol, 1e-4)
+        assert abs(vx - vy) <= tol, (k, x, y, tol)
+
+
 def test_mixed_program_matches_oracle(tmp_path):
     """The drop-in program on its default (mixed) path: 12 macro records
     against the oracle.  Columns that integrate the field (kinetic
-    energies, momenta, speeds) agree to 1e-5 relative; the potential
+    energies, speeds) agree to 1e-5 relative, the species momenta to 1e-5 of
+    N m v_th (and R N m v_th for the angular momenta); the potential
     energies, computed by the float64 walk from positions that moved by
     dt/2 times the velocity difference, agree to 1e-5; the grain charge
     and the absorbed-electron count are exact."""
     import struct
     from make_golden import splitmix64_words
-    from test_gpu_host import case_args, close_lines, oracle_macro_lines, run_pot
+    from test_gpu_host import case_args, oracle_macro_lines, run_pot
     case = big_case(2001, seed=5)
     seed = 4242
     (tmp_path / "seed.bin").write_bytes(struct.pack("<16I", *splitmix64_words(seed)))
@@ -135,5 +160,15 @@ def test_mixed_program_matches_oracle(tmp_path):
     got = open(out / "pot-macroscopic.dat").read().splitlines(True)
     assert len(got) == steps + 1
     want = oracle_macro_lines(case, seed, steps)
+    N = int(case["args"]["-N"])
+    R = domain_radius(case)
+    kb = 1.38065e-23
+    pe = (N - 1) / 2 * ME * np.sqrt(kb * float(case["args"]["-E"]) / ME)
+    pi = (N - 1) / 2 * 1836.15 * ME * np.sqrt(kb * float(case["args"]["-I"]) / (1836.15 * ME))
+    scale = {k: pe for k in (22, 23, 24)}
+    scale.update({k: pi for k in (25, 26, 27)})
+    scale.update({k: R * pe for k in (28, 29, 30)})
+    scale.update({k: R * pi for k in (31, 32, 33)})
+    mom_atol = {k: 1e-5 * v for k, v in scale.items()}
     for t in range(steps):
-        close_lines(got[t + 1], want[t], rel=1e-5)
+        _close_mixed(got[t + 1], want[t], 1e-5, mom_atol)
