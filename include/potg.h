@@ -224,6 +224,18 @@ int potg_field_at(potg_ctx* ctx, uint32_t inst, const double* xyz, const uint32_
                   size_t n, double* E, uint32_t* counts);
 /* Potential at n points (phi_at, pot.c:1818-1901). */
 int potg_phi_at(potg_ctx* ctx, uint32_t inst, const double* xyz, size_t n, double* phi);
+/* potg_phi_at for callers that cannot allocate: code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc.  No
+   counterpart in the reference.  d_xyz (3 n doubles) and d_phi (n doubles)
+   are device memory; the call allocates nothing (its scratch is a fixed
+   array of about 0.5 MB per device, held until the process ends), does not
+   synchronise and enqueues on `stream` (a hipStream_t; NULL for the
+   context's stream).  n may be at most POTG_NOALLOC_MAX_POINTS, else
+   POTG_ERR_ARG.  Calls that overlap on one GPU share the scratch and race.
+   Results equal potg_phi_at's; errors a kernel raises surface at potg_sync. */
+#define POTG_NOALLOC_MAX_POINTS 65536u
+int potg_phi_at_noalloc(potg_ctx* ctx, uint32_t inst, const double* d_xyz, size_t n,
+                        double* d_phi, void* stream);
 /* Softened float64 direct sum at the positions of particles idx[0..n) of
    instance inst over all its other particles (E_ref of the force checks). */
 int potg_direct_sum(potg_ctx* ctx, uint32_t inst, const uint32_t* idx, size_t n,

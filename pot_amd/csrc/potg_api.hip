@@ -333,6 +333,9 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     b.n_inst = n_inst;
     b.precision = precision;
     b.sort_ipt = 16;
+    // this device's scratch of potg_phi_at_noalloc (its address, once per
+    // device; the array itself is static)
+    if (noalloc_scratch(&b.noalloc)) b.noalloc = nullptr;
     for (uint32_t i = 0; i < n_inst; i++) b.any_spheroid |= configs[i].geometry != POTG_SPHERE;
     b.count = 1;
     int ib = 0;
@@ -837,6 +840,23 @@ int potg_phi_at(potg_ctx* c, uint32_t inst, const double* xyz, size_t n, double*
     if (!c || inst >= c->b.n_inst || (n && (!xyz || !phi)))
         return set_err("potg_phi_at: bad arguments"), POTG_ERR_ARG;
     return run_points(c, inst, xyz, nullptr, n, 1, phi, nullptr);
+}
+
+int potg_phi_at_noalloc(potg_ctx* c, uint32_t inst, const double* d_xyz, size_t n, double* d_phi,
+                        void* stream)
+{
+    if (!c || inst >= c->b.n_inst || (n && (!d_xyz || !d_phi)))
+        return set_err("potg_phi_at_noalloc: bad arguments"), POTG_ERR_ARG;
+    if (n > POTG_NOALLOC_MAX_POINTS)
+        return set_err("potg_phi_at_noalloc: %zu points, more than the %u its scratch holds", n,
+                       POTG_NOALLOC_MAX_POINTS),
+               POTG_ERR_ARG;
+    if (!c->b.noalloc) return set_err("potg_phi_at_noalloc: no scratch on this device"), POTG_ERR_HIP;
+    if (n == 0) return POTG_OK;
+    launch_phi_noalloc(c->b, c->b.noalloc, inst, d_xyz, (uint32_t)n, d_phi,
+                       stream ? (hipStream_t)stream : c->stream);
+    HIPCHK(hipGetLastError());
+    return POTG_OK;
 }
 
 int potg_direct_sum(potg_ctx* c, uint32_t inst, const uint32_t* idx, size_t n, double* E)

@@ -86,6 +86,7 @@ struct Bufs {
     double* q_out;
     uint32_t* q_counts;
     uint32_t q_cap;
+    uint32_t* noalloc;  // this device's potg_phi_at_noalloc scratch
 };
 
 // step stages (kernel launches only; no synchronisation)
@@ -117,5 +118,10 @@ void launch_unsort(Bufs& b, hipStream_t s);
 // point queries: n points of q_pts with q_inst/q_self; kind 0 field, 1 potential
 void launch_points(Bufs& b, uint32_t n, int kind, hipStream_t s);
 void launch_direct_sum(Bufs& b, uint32_t inst, uint32_t n, hipStream_t s);
+// potg_phi_at_noalloc: this device's scratch array (0 on success), and the
+// point-potential walk over the caller's device arrays using it
+int noalloc_scratch(uint32_t** out);
+void launch_phi_noalloc(Bufs& b, uint32_t* scr, uint32_t inst, const double* pts, uint32_t n,
+                        double* out, hipStream_t s);
 
 }  // namespace potg

@@ -21,6 +21,8 @@
 // domain centre in units of R, float32 arithmetic with a hardware reciprocal
 // square root, Kahan-compensated per-particle sums, result scaled back in
 // float64.
+#include <atomic>
+
 #include "potg_grain.h"
 #include "potg_internal.h"
 
@@ -1588,6 +1590,72 @@ __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, 
 void launch_chi_b(Bufs& b, hipStream_t s)
 {
     k_chi_b<<<b.n_inst, 64, 0, s>>>(b.P, b.S, b.g, b.rec64, b.dip64, b.n_inst, b.n_rec);
+}
+
+// ---- point potentials with no allocation (potg_phi_at_noalloc) -----------
+//
+// The walk of a point query reads, besides the caller's points and result
+// array, a tile counter and an instance and excluded-slot entry per point.
+// For callers that cannot allocate, that scratch is this array: word 0 is
+// the tile counter, then NOALLOC_MAX points' instances, then their excluded
+// slots (all NONE: a potential counts every leaf).  Each device holds one
+// copy until the process ends; the host reaches it only through the address
+// hipGetSymbolAddress gives on that device.  Calls that overlap on one device
+// share it and race.
+constexpr uint32_t NOALLOC_HEAD = 64;  // words before the per-point arrays
+constexpr uint32_t NOALLOC_WORDS = NOALLOC_HEAD + 2 * POTG_NOALLOC_MAX_POINTS;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+
+__global__ void k_noalloc_prep(uint32_t* __restrict__ scr, uint32_t inst, uint32_t n)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) scr[0] = 0u;  // the walk's tile counter starts from zero
+    if (k < n) {
+        scr[NOALLOC_HEAD + k] = inst;
+        scr[NOALLOC_HEAD + POTG_NOALLOC_MAX_POINTS + k] = NONE;
+    }
+}
+
+}  // namespace
+
+int noalloc_scratch(uint32_t** out)
+{
+    // the address of this device's copy, looked up once per device
+    constexpr int MAXDEV = 64;
+    static std::atomic<uint32_t*> addr[MAXDEV];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV) return -1;
+    uint32_t* p = addr[dev].load();
+    if (!p) {
+        void* q = nullptr;
+        if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return -1;
+        p = (uint32_t*)q;
+        addr[dev].store(p);
+    }
+    *out = p;
+    return 0;
+}
+
+void launch_phi_noalloc(Bufs& b, uint32_t* scr, uint32_t inst, const double* pts, uint32_t n,
+                        double* out, hipStream_t s)
+{
+    WalkIO io{};
+    io.pts = pts;
+    io.pt_inst = scr + NOALLOC_HEAD;
+    io.pt_self = scr + NOALLOC_HEAD + POTG_NOALLOC_MAX_POINTS;
+    io.n_pts = n;
+    io.out = out;
+    io.acc = 1;
+    io.tile_ctr = scr;
+    io.n_rec = b.n_rec;
+    io.dip64 = b.dip64;
+    const uint32_t nb = (n + BT - 1) / BT ? (n + BT - 1) / BT : 1;
+    const int blocks = (int)nb < (int)b.walk_blocks ? (int)nb : (int)b.walk_blocks;
+    k_noalloc_prep<<<nb, BT, 0, s>>>(scr, inst, n);
+    k_grain_init<MODE_POINTS, KIND_PHI><<<nb, BT, 0, s>>>(b.P, b.S, b.st, io);
+    launch_walk64<MODE_POINTS, KIND_PHI>(b, blocks, io, s);
 }
 
 }  // namespace potg

@@ -16,6 +16,7 @@ POTG_SPHERE, POTG_PROLATE, POTG_OBLATE = 0, 1, 2
 POTG_FP64, POTG_MIXED, POTG_MIXED_GROUP = 0, 1, 2
 POTG_BORIS, POTG_VERLET, POTG_EULER_RICHARDSON = 0, 1, 2
 POTG_ERR_ARG = -1
+POTG_NOALLOC_MAX_POINTS = 65536  # potg_phi_at_noalloc's limit (include/potg.h)
 DUST, ION, ELECTRON = 0, 1, 2
 
 PARTICLE_DT = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("vx", "<f8"),
@@ -30,7 +31,8 @@ EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "pot
            "potg_step", "potg_run", "potg_build_tree", "potg_tree_canon", "potg_field_at",
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
            "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles", "potg_tune",
-           "potg_set_tuning", "potg_set_counting", "potg_check_particles"]
+           "potg_set_tuning", "potg_set_counting", "potg_check_particles",
+           "potg_phi_at_noalloc"]
 
 
 class Config(C.Structure):
@@ -107,6 +109,7 @@ def lib():
     L.potg_tune.argtypes = [P, C.POINTER(Tuning)]
     L.potg_set_tuning.argtypes = [P, C.POINTER(Tuning), C.POINTER(Tuning)]
     L.potg_check_particles.argtypes = [C.POINTER(Config), P, C.POINTER(C.c_uint64)]
+    L.potg_phi_at_noalloc.argtypes = [P, C.c_uint32, P, C.c_size_t, P, P]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
