@@ -9,8 +9,9 @@ The PRNG seed is the 64-byte file ./seed.bin (RANDOMNESS_SOURCE, pot.c:192)
 expanded from an integer by splitmix64, so a test can rebuild it.
 
 Only this container has the reference; the fixtures travel, the reference
-does not.  Usage: python tests/golden/make_golden.py [--ensemble]
-(--ensemble: only the 2000-step runs of the CPU-runnable workload, below)
+does not.  Usage: python tests/golden/make_golden.py [--ensemble | --paper]
+(--ensemble: only the 2000-step runs of the CPU-runnable workload; --paper:
+the grain-charge series of the reference's published run, below)
 """
 import json
 import math
@@ -187,7 +188,33 @@ def ensemble():
     print("wrote", dest)
 
 
+# The grain-charge series of the reference's published run with
+# particle-particle forces in a spherical domain (output/2016 TH PPCF
+# paper/np-with-sph-macro.dat; gasma.R:119-127 plots it): TIME and DGQ only.
+PAPER_SRC = os.path.join("/root/reference", "output", "2016 TH PPCF paper",
+                         "np-with-sph-macro.dat")
+
+
+def paper():
+    dest = os.path.join(HERE, "paper_2016")
+    os.makedirs(dest, exist_ok=True)
+    lines = open(PAPER_SRC).read().split("\n")
+    head = lines[0].split()
+    it, iq = head.index("TIME"), head.index("DGQ")
+    out = ["TIME DGQ"]
+    for ln in lines[1:]:
+        f = ln.split()
+        if f:
+            out.append("%s %s" % (f[it], f[iq]))
+    with open(os.path.join(dest, "np_with_sph_dgq.dat"), "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    print("wrote", dest, len(out) - 1, "records")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--paper":
+        paper()
+        return
     if not os.path.isdir(REF_BIN):
         sys.exit("build the reference variants first: oracle/build_ref.sh")
     if len(sys.argv) > 1 and sys.argv[1] == "--ensemble":
