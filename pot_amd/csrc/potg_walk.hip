@@ -675,8 +675,12 @@ __device__ inline uint32_t sweep_range(const Staged<DIPK>* sr, uint32_t tc, uint
     const uint32_t lane = __lane_id();
     const bool act = (tm >> lane) & 1ull;
     const uint32_t all = (1u << tc) - 1u;
-    for (uint32_t m = all & lm; m; m &= m - 1u) {
-        const uint32_t j = (uint32_t)__builtin_ctz(m);
+    const uint32_t lv = all & lm, cl = all & ~lm;
+    // unrolled over the eight child slots (a scalar test skips a slot), so
+    // that each slot's LDS reads take a constant offset from one base
+#pragma unroll
+    for (uint32_t j = 0; j < 8u; j++) {
+        if (!((lv >> j) & 1u)) continue;
         const NodeRec32 r = sr[j].r;
         const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -691,9 +695,10 @@ __device__ inline uint32_t sweep_range(const Staged<DIPK>* sr, uint32_t tc, uint
     }
     uint32_t opened = 0;
     first_om = 0;
-    for (uint32_t c = all & ~lm; c;) {
-        const uint32_t j = 31u - (uint32_t)__builtin_clz(c);
-        c &= ~(1u << j);
+#pragma unroll
+    for (int jj = 7; jj >= 0; jj--) {
+        const uint32_t j = (uint32_t)jj;
+        if (!((cl >> j) & 1u)) continue;
         const NodeRec32 r = sr[j].r;
         const float dx = tx - r.cx, dy = ty - r.cy, dz = tz - r.cz;
         const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -783,17 +788,20 @@ __global__ __launch_bounds__(BT) void k_walk32s(const InstParams* __restrict__ P
                 tot_pops++;
                 uint32_t lm = 0, hqm = 0;
                 {
-                    NodeRec32 mr;
-                    if (lane < tc) mr = rec[tf + lane];
-                    else mr = NodeRec32{0.f, 0.f, 0.f, 0.f, 0.f, 0u, 0u, 0u};
+                    // lane j stages child j; only lanes holding a child
+                    // touch the record, so nothing else is materialised
+                    bool leaf = false, hq = false;
                     if (lane < tc) {
+                        const NodeRec32 mr = rec[tf + lane];
                         s_rec[wv][lane].r = mr;
                         if constexpr (DIP) {
                             if (dip) s_rec[wv][lane].d[0] = io.dip32[tf + lane];
                         }
+                        leaf = mr.cnt == LEAF;
+                        hq = (DIP && dip) ? mr.part != 0u : mr.q != 0.f;
                     }
-                    lm = (uint32_t)ballot(lane < tc && mr.cnt == LEAF);
-                    hqm = (uint32_t)ballot(lane < tc && ((DIP && dip) ? mr.part != 0u : mr.q != 0.f));
+                    lm = (uint32_t)ballot(leaf);
+                    hqm = (uint32_t)ballot(hq);
                 }
                 __builtin_amdgcn_wave_barrier();
                 float px = 0.f, py = 0.f, pz = 0.f;
