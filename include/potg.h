@@ -146,7 +146,11 @@ typedef struct potg_stats_t {
                                 is the fraction of lanes doing an interaction */
 } potg_stats_t;
 
-/* Launch parameters of the field walk (potg_tune picks them by timing). */
+/* Launch parameters of the field walk (potg_tune picks them by timing).
+   Until it or potg_set_tuning runs they follow the context's size: from
+   about 0.5M particles 64 targets per wavefront (sweeping whole ranges in
+   the float32 walk) and 8 blocks per CU, below that 16 targets per
+   wavefront (64 for the float64 walk) and 4 blocks per CU. */
 typedef struct potg_tuning {
     uint32_t walk_blocks; /* blocks (4 wavefronts each) of the persistent walk */
     uint32_t group;       /* float32 walk: targets per wavefront, 8, 16 or 64;

@@ -345,15 +345,15 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     // defaults until potg_tune or potg_set_tuning, by size: with enough
     // 64-target tiles to give every SIMD several wavefronts (about 0.5M
     // particles and up, half of them stepped each step), 64 targets per
-    // wavefront and 8 blocks of 4 waves per CU; below that the tiles would
-    // leave most of the chip idle, so 16 targets per wavefront and 4
-    // blocks per CU
+    // wavefront sweeping whole ranges and 8 blocks of 4 waves per CU; below
+    // that the tiles would leave most of the chip idle, so 16 targets per
+    // wavefront and 4 blocks per CU
     {
         const uint64_t tiles = n_total / 128u;
         const bool large = tiles >= 16u * (uint64_t)device_cus();
         b.walk_blocks = (large ? 8u : 4u) * (uint32_t)device_cus();
         b.group = (large || precision == POTG_FP64) ? 64u : 16u;  // float64: always 64
-        b.variant = 0;
+        b.variant = (large && precision == POTG_MIXED) ? 1u : 0u;
     }
 
     c->hp.resize(n_inst);
