@@ -9,7 +9,7 @@ percentile 2e-3: MAC decisions that flip at the threshold in float32)."""
 import numpy as np
 import pytest
 
-from gpu_common import CASES, big_case, gpu_ctx, oracle_sim
+from gpu_common import CASES, big_case, gpu_config, gpu_ctx, oracle_sim
 
 from pot_amd import potg as G
 
@@ -76,3 +76,26 @@ def test_sweep_steps_conserve_and_count(dipoles):
     for k in ("n_cell", "n_leaf", "n_open"):
         assert s0[k] == s1[k], k
     assert sc["steps"] == sa["steps"] == 3
+
+
+def test_untuned_shape_follows_size():
+    """Until potg_tune or potg_set_tuning the walk's shape follows the
+    context's size: from 16 64-target tiles per CU (about 0.5M particles
+    with the 4-block-per-CU rule) the float32 walk sweeps whole ranges with
+    64 targets per wavefront on 8 blocks per CU; below, 16 targets per
+    wavefront on 4 blocks per CU; the float64 walk always takes 64."""
+    small = gpu_config(big_case(20001))
+    big = gpu_config(big_case(1500001))
+    shapes = {}
+    for name, cfg in (("small", small), ("big", big)):
+        for prec in (G.POTG_MIXED, G.POTG_FP64):
+            ctx = G.Context([cfg], prec)
+            shapes[name, prec] = ctx.tuning()
+            ctx.close()
+    b_mixed, b_fp64 = shapes["big", G.POTG_MIXED], shapes["big", G.POTG_FP64]
+    s_mixed, s_fp64 = shapes["small", G.POTG_MIXED], shapes["small", G.POTG_FP64]
+    assert (b_mixed["group"], b_mixed["variant"]) == (64, 1)
+    assert b_fp64["group"] == 64 and b_fp64["walk_blocks"] == b_mixed["walk_blocks"]
+    assert (s_mixed["group"], s_mixed["variant"]) == (16, 0)
+    assert s_fp64["group"] == 64
+    assert b_mixed["walk_blocks"] == 2 * s_mixed["walk_blocks"]
