@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "../../include/potg.h"
@@ -80,6 +81,17 @@ int alloc_n(potg_ctx* c, T** p, size_t n)
     return dev_alloc(c, (void**)p, n * sizeof(T));
 }
 
+// "instance i, particle k" for a global particle slot (instances are
+// concatenated, each from its offset; k is the reference's index)
+std::string slot_name(const potg_ctx* c, uint32_t idx)
+{
+    for (size_t i = 0; i < c->hp.size(); i++)
+        if (idx >= c->hp[i].offset && idx - c->hp[i].offset < c->hp[i].N)
+            return "instance " + std::to_string(i) + ", particle " +
+                   std::to_string(idx - c->hp[i].offset);
+    return "global slot " + std::to_string(idx);
+}
+
 int kernel_error(potg_ctx* c)
 {
     int32_t err = 0;
@@ -92,17 +104,17 @@ int kernel_error(potg_ctx* c)
     HIPCHK(hipMemcpy(&c->b.g->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
     switch (err) {
     case KERR_OUT_OF_BOX:
-        set_err("particle (global slot %u) not in simulation area", idx);
+        set_err("particle (%s) not in simulation area", slot_name(c, idx).c_str());
         return POTG_ERR_OUT_OF_BOX;
     case KERR_COINCIDENT:
-        set_err("multiple particles at one location (slot %u)", idx);
+        set_err("multiple particles at one location (%s)", slot_name(c, idx).c_str());
         return POTG_ERR_COINCIDENT;
     case KERR_CAPACITY:
         set_err("particle tree larger than its preallocated capacity (%u)", idx);
         return POTG_ERR_NOMEM;
     case KERR_REINJECT:
-        set_err("particle (global slot %u) found no admissible SCEPTIC reinjection in 1000 "
-                "tries: chi_b bars its species from entering", idx);
+        set_err("particle (%s) found no admissible SCEPTIC reinjection in 1000 "
+                "tries: chi_b bars its species from entering", slot_name(c, idx).c_str());
         return POTG_ERR_REINJECT;
     default:
         set_err("particle tree path length exceeded (node %u)", idx);

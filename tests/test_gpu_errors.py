@@ -79,3 +79,31 @@ def test_nan_arising_on_the_device_is_reported(precision):
     with pytest.raises(G.PotgError, match="POTG_ERR_OUT_OF_BOX"):
         ctx.step()
     _recovers(ctx, parts)
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_errors_name_the_instance_and_particle(precision):
+    """In a batched context the error names the instance and the reference's
+    particle index within it, not a slot of the concatenated state."""
+    from gpu_common import gpu_config
+    parts = _valid()
+    cfg = gpu_config(CASE)
+    ctx = G.Context([cfg, cfg], precision)
+    bad = parts.copy()
+    for f in ("x", "y", "z"):
+        bad[f][1500] = bad[f][700]
+    ctx.upload(0, parts)
+    ctx.upload(1, bad)
+    with pytest.raises(G.PotgError, match=r"POTG_ERR_COINCIDENT.*instance 1, particle (700|1500)\)"):
+        ctx.step()
+    nan = parts.copy()
+    j = int(np.nonzero(nan["species"] == G.ELECTRON)[0][10])
+    nan["vx"][j] = np.nan
+    ctx.upload(0, parts)
+    ctx.upload(1, nan)
+    ctx.step()
+    with pytest.raises(G.PotgError, match=r"POTG_ERR_OUT_OF_BOX.*instance 1, particle %d\)" % j):
+        ctx.step()
+    ctx.upload(1, parts)
+    ctx.upload(0, parts)
+    assert ctx.step()[0]["n_pushed"] > 0
