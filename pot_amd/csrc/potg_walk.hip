@@ -673,7 +673,9 @@ __device__ inline uint32_t sweep_range(const Staged<DIPK>* sr, uint32_t tc, uint
                                        uint32_t& wopen, uint64_t& first_om)
 {
     const uint32_t lane = __lane_id();
-    const bool act = (tm >> lane) & 1ull;
+    // the lane's bit of the mask, used as a lane mask directly (no vector
+    // shift and compare)
+    const bool act = __builtin_amdgcn_inverse_ballot_w64(tm);
     const uint32_t all = (1u << tc) - 1u;
     const uint32_t lv = all & lm, cl = all & ~lm;
     // unrolled over the eight child slots (a scalar test skips a slot), so
@@ -711,10 +713,10 @@ __device__ inline uint32_t sweep_range(const Staged<DIPK>* sr, uint32_t tc, uint
         if (D) {
             const float4 pd = sr[j].d[0];
             const float rpd2 = __builtin_fmaf(dx, pd.x, __builtin_fmaf(dy, pd.y, dz * pd.z)) * inv2;
-            const float ca = inv3 * __builtin_fmaf(3.f, rpd2, r.q);
-            px = __builtin_fmaf(-inv3, pd.x, __builtin_fmaf(ca, dx, px));
-            py = __builtin_fmaf(-inv3, pd.y, __builtin_fmaf(ca, dy, py));
-            pz = __builtin_fmaf(-inv3, pd.z, __builtin_fmaf(ca, dz, pz));
+            const float ca = __builtin_fmaf(3.f, rpd2, r.q);
+            px = __builtin_fmaf(inv3, __builtin_fmaf(ca, dx, -pd.x), px);
+            py = __builtin_fmaf(inv3, __builtin_fmaf(ca, dy, -pd.y), py);
+            pz = __builtin_fmaf(inv3, __builtin_fmaf(ca, dz, -pd.z), pz);
         } else {
             const float ca = r.q * inv3;
             px = __builtin_fmaf(ca, dx, px);
@@ -788,20 +790,20 @@ __global__ __launch_bounds__(BT) void k_walk32s(const InstParams* __restrict__ P
                 tot_pops++;
                 uint32_t lm = 0, hqm = 0;
                 {
-                    // lane j stages child j; only lanes holding a child
-                    // touch the record, so nothing else is materialised
-                    bool leaf = false, hq = false;
-                    if (lane < tc) {
-                        const NodeRec32 mr = rec[tf + lane];
+                    // lane j stages child j; the lanes past the range read
+                    // child 0 again (a clamped index: the load and the ballots
+                    // need no branch) and their bits are masked off
+                    const bool in = lane < tc;
+                    const NodeRec32 mr = rec[tf + (in ? lane : 0u)];
+                    if (in) {
                         s_rec[wv][lane].r = mr;
                         if constexpr (DIP) {
                             if (dip) s_rec[wv][lane].d[0] = io.dip32[tf + lane];
                         }
-                        leaf = mr.cnt == LEAF;
-                        hq = (DIP && dip) ? mr.part != 0u : mr.q != 0.f;
                     }
-                    lm = (uint32_t)ballot(leaf);
-                    hqm = (uint32_t)ballot(hq);
+                    const uint32_t inm = (1u << tc) - 1u;
+                    lm = (uint32_t)ballot(mr.cnt == LEAF) & inm;
+                    hqm = (uint32_t)ballot((DIP && dip) ? mr.part != 0u : mr.q != 0.f) & inm;
                 }
                 __builtin_amdgcn_wave_barrier();
                 float px = 0.f, py = 0.f, pz = 0.f;
