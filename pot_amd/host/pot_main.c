@@ -337,6 +337,38 @@ static int record_macro(const pot_run* ru, unsigned long t)
     return rec;
 }
 
+/* the run's wall-time limit (-w, minutes) has passed */
+static int over_time(const pot_run* ru, time_t start)
+{
+    if (!(ru->tim_lim > 0.0)) return 0;
+    const time_t now = time(NULL);
+    if (now < start) {
+        fputs("0: system time seems to've gone backwards, wall-time limit unenforceable!\n",
+              stderr);
+        return 0;
+    }
+    return (now - start) > (60 * ru->tim_lim);
+}
+
+/* Steps t+1, t+2, ... (at most PLAIN_MAX) before the next one that writes a
+   macro record (record_macro), a state file (every 50000 steps, before the
+   step) or the phi file (every 20000, after it), is tuned (--tune, step 3)
+   or may warn (OUT_BOUNCE), or that passes -i. */
+#define PLAIN_MAX 200ul
+static unsigned long plain_steps(const pot_run* ru, unsigned long t)
+{
+    if (ru->out_bounce) return 0;
+    unsigned long k = 0;
+    while (k < PLAIN_MAX) {
+        const unsigned long s = t + 1 + k;
+        if ((ru->run_for && s > ru->run_for) || record_macro(ru, s) ||
+            (ru->sw_path && !(s % 50000)) || !(s % 20000) || (ru->tune && s <= 3))
+            break;
+        k++;
+    }
+    return k;
+}
+
 /* key of the counter-based reinjection stream: a mix of the WELL512 record,
    so a run resumed from its state file draws the same numbers */
 static uint64_t stream_key(const pot_prng* ps)
@@ -503,16 +535,27 @@ static int run(host_run* h, uint64_t t_completed)
         pot_write_run_description(path, ru, &def);
     }
     while (1) {
+        /* the coming steps after which nothing is written, tuned or warned
+           of run as replays of a captured step, with no host round trip
+           between them; they end where such a step comes */
+        const unsigned long k = plain_steps(ru, t);
+        if (k >= 2) {
+            if (over_time(ru, start)) {
+                t++;  /* as the one-step path leaves it: the step not taken */
+                break;
+            }
+            if (potg_run(h->ctx, k) != POTG_OK) {
+                fprintf(stderr, "0:%lu-%lu: %s\n", t + 1, t + k, potg_last_error());
+                failed = 1;
+                break;
+            }
+            t += k;
+            rec = 0;
+            continue;
+        }
         t++;
         if (ru->run_for && t > ru->run_for) break;
-        if (ru->tim_lim > 0.0) {
-            const time_t now = time(NULL);
-            if (now < start)
-                fputs("0: system time seems to've gone backwards, wall-time limit unenforceable!\n",
-                      stderr);
-            else if ((now - start) > (60 * ru->tim_lim))
-                break;
-        }
+        if (over_time(ru, start)) break;
         if (ru->sw_path && !(t % 50000)) {
             if (fetch(h)) {
                 failed = 1;

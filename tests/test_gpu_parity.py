@@ -470,3 +470,33 @@ def test_two_field_integrators_over_steps(name):
         d = np.abs(pg[f] - po[f])
         assert d[~moved].max() <= 1e-9 * vscale, f
     assert gr["q"] == sim.state()["q"]
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_replays_interleaved_with_eager_steps(precision):
+    """The pot program runs the steps between its outputs as replays and the
+    others one by one: any interleaving of the two ends where eager steps
+    end, across a chi_b step (every 20 steps) and steps with the potential
+    energies (compute_epe)."""
+    case = big_case(20001, seed=23)
+    parts = oracle_sim(case).particles()
+    a = gpu_ctx(case, parts, precision=precision, seed=7)
+    b = gpu_ctx(case, parts, precision=precision, seed=7)
+    for c in (a, b):
+        c.set_counting(False)
+    plan = [("step", 1), ("step", 0), ("run", 4), ("step", 1), ("run", 17), ("step", 0),
+            ("run", 2), ("step", 1)]
+    for kind, arg in plan:
+        if kind == "step":
+            a.step(arg)
+            b.step(arg)
+        else:
+            for _ in range(arg):
+                a.step(0)
+            b.run(arg)
+    assert sum(n if k == "run" else 1 for k, n in plan) == 28
+    pa, ga = a.download(0)
+    pb, gb = b.download(0)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "species"):
+        assert np.array_equal(pa[f], pb[f]), f
+    assert ga == gb
