@@ -961,6 +961,8 @@ int potg_set_tuning(potg_ctx* c, const potg_tuning* set, potg_tuning* get)
     return POTG_OK;
 }
 
+constexpr double TUNE_MARGIN = 1.01;
+
 int potg_tune(potg_ctx* c, potg_tuning* out)
 {
     if (!c) return set_err("potg_tune: null context"), POTG_ERR_ARG;
@@ -985,7 +987,7 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     const uint32_t shapes[][2] = {{8, 0}, {16, 0}, {64, 0}, {64, 1}};
     const uint32_t bpc[] = {4, 8, 16};
     const uint32_t keep_wb = b.walk_blocks, keep_gr = b.group, keep_var = b.variant;
-    double best = 1e30;
+    double best = 1e30, keep_ms = -1.0;
     uint32_t best_wb = keep_wb, best_gr = keep_gr, best_var = keep_var;
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     for (uint32_t wb : bpc) {
@@ -1009,6 +1011,7 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
                 HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                 if (ms < ms_best) ms_best = ms;
             }
+            if (b.walk_blocks == keep_wb && gr == keep_gr && sh[1] == keep_var) keep_ms = ms_best;
             if (ms_best < best) {
                 best = ms_best;
                 best_wb = b.walk_blocks;
@@ -1017,13 +1020,21 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
             }
         }
     }
+    // the shape in use stays unless another is faster by more than the
+    // timing noise (1%): equivalent shapes do not flap from run to run
+    if (keep_ms >= 0.0 && keep_ms <= TUNE_MARGIN * best) {
+        best = keep_ms;
+        best_wb = keep_wb;
+        best_gr = keep_gr;
+        best_var = keep_var;
+    }
     // the radix sort's tile: the keys, sort and permute of the coming step
     // again under each tile size (the state is sorted already, so the order
     // stays as it is)
     const uint32_t keep_ipt = b.sort_ipt;
     uint32_t best_ipt = keep_ipt;
     if (!rc) {
-        float sbest = 1e30f;
+        float sbest = 1e30f, skeep = -1.f;
         for (uint32_t ipt : {8u, 16u}) {
             b.sort_ipt = ipt;
             float ms_best = 1e30f;
@@ -1036,11 +1047,13 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
                 HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                 if (ms < ms_best) ms_best = ms;
             }
+            if (ipt == keep_ipt) skeep = ms_best;
             if (ms_best < sbest) {
                 sbest = ms_best;
                 best_ipt = ipt;
             }
         }
+        if (skeep >= 0.f && skeep <= TUNE_MARGIN * sbest) best_ipt = keep_ipt;
         rc = kernel_error(c);
     }
     b.sort_ipt = rc ? keep_ipt : best_ipt;
