@@ -256,6 +256,20 @@ int potg_set_tuning(potg_ctx* ctx, const potg_tuning* set, potg_tuning* get);
 
 int potg_stats(potg_ctx* ctx, potg_stats_t* out);
 int potg_stats_reset(potg_ctx* ctx);
+/* What a SCEPTIC reinjection that finds no admissible entry does (the
+   reference recurses without end there, pot.c:583-598).  Off (0, the
+   default): the step returns POTG_ERR_REINJECT.  On: the particle's
+   instance retires -- the particle is parked at rest inside the grain and,
+   from the next step on, none of the instance's particles is stepped and
+   its chi_b is no longer measured -- and the other instances go on.  For
+   parameter sweeps, where one instance's failure should not end the batch.
+   No counterpart in the reference. */
+int potg_set_retire(potg_ctx* ctx, int32_t on);
+/* Whether instance inst has retired (*retired = 1) and the step in which
+   its reinjection failed (*step, may be NULL; 0 when not retired).
+   potg_upload of the instance clears it. */
+int potg_instance_retired(potg_ctx* ctx, uint32_t inst, int32_t* retired, uint64_t* step);
+
 /* Interaction counters (n_cell, n_leaf, n_open, n_range_loads) of the step
    walks on (on != 0, the default) or off.  They are instrumentation: off,
    the float32 walk runs 10-16% faster; point queries count either way.

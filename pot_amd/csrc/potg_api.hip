@@ -924,6 +924,28 @@ int potg_set_counting(potg_ctx* c, int32_t on)
     return POTG_OK;
 }
 
+int potg_set_retire(potg_ctx* c, int32_t on)
+{
+    if (!c) return set_err("potg_set_retire: null context"), POTG_ERR_ARG;
+    const uint32_t v = on != 0;
+    // a device flag: captured steps read it, so they stay valid
+    HIPCHK(hipMemcpyAsync(&c->b.g->retire, &v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return POTG_OK;
+}
+
+int potg_instance_retired(potg_ctx* c, uint32_t inst, int32_t* retired, uint64_t* step)
+{
+    if (!c || inst >= c->b.n_inst || !retired)
+        return set_err("potg_instance_retired: bad arguments"), POTG_ERR_ARG;
+    InstState s;
+    HIPCHK(hipMemcpyAsync(&s, &c->b.S[inst], sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *retired = s.retired ? 1 : 0;
+    if (step) *step = s.retired ? (uint64_t)s.retired_t : 0;
+    return POTG_OK;
+}
+
 int potg_stats_reset(potg_ctx* c)
 {
     if (!c) return set_err("potg_stats_reset: null context"), POTG_ERR_ARG;

@@ -97,6 +97,11 @@ struct InstState {
     // outputs of the last step
     int32_t out_q;
     uint32_t out_e, out_i, out_pushed, out_reinj_abs, out_reinj_out, out_wall_inward;
+    // retired by a failed reinjection (potg_set_retire): no longer stepped;
+    // retired_t is the step in which it failed
+    uint32_t retired;
+    uint32_t retired_pad;
+    unsigned long long retired_t;
 };
 
 // Context-wide device counters.
@@ -116,6 +121,8 @@ struct Globals {
     unsigned long long n_cell, n_leaf, n_open, n_tgt_total, n_steps;
     unsigned long long n_pops;   // child ranges fetched by the walk (per wavefront)
     unsigned long long n_visits; // children visited by float32-walk wavefronts
+    uint32_t retire;             // a failed reinjection retires its instance (1) or
+                                 // fails the step (0, the default)
 };
 
 // Tree node records, children contiguous: an internal node's children are

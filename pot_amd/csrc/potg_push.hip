@@ -605,9 +605,26 @@ __global__ __launch_bounds__(BT) void k_reinject(const InstParams* __restrict__ 
                 else plain_reinject(p, s, rs, x, y, z, vx, vy, vz);
             }
         }
-        if (!ok) {
+        if (!ok && !g->retire) {
             if (lane == 0) raise_err(g, KERR_REINJECT, st.ref[i]);
             continue;  // the particle keeps its state; the step reports the error
+        }
+        if (!ok) {
+            // the instance retires: from the next step on none of its
+            // particles is a target, so nothing of it moves again.  The
+            // particle is parked at rest inside the grain, on the x axis at
+            // a distance that its index makes unique (half the grain's
+            // radius at most: inside the sphere and both spheroids), so the
+            // instance's tree still builds
+            if (lane == 0) {
+                S[inst].retired = 1u;
+                S[inst].retired_t = g->t;
+                const double local = (double)(st.ref[i] - p.offset);
+                x = p.R + 0.5 * p.a * (local + 1.0) / ((double)p.N + 1.0);
+                y = p.R;
+                z = p.R;
+                vx = vy = vz = 0.0;
+            }
         }
         if (lane == 0) {
             st.x[i] = x;
@@ -627,7 +644,7 @@ __global__ void k_target_flags(const InstParams* __restrict__ P, const InstState
     if (i >= n_tree) return;
     const uint32_t inst = st.inst[i];
     const uint32_t local = st.ref[i] - P[inst].offset;
-    flags[i] = (st.spec[i] != 0 && local >= S[inst].start) ? 1u : 0u;
+    flags[i] = (st.spec[i] != 0 && local >= S[inst].start && !S[inst].retired) ? 1u : 0u;
 }
 
 // The step's targets grouped by instance, each instance's run starting at a

@@ -66,12 +66,13 @@ __device__ inline int split_depth(double ax, double ay, double az, double bx, do
 }
 
 // ---------------------------------------------------------------------
+// One block: every thread reads the step counter before thread 0 advances
+// it (the barrier), however many instances there are.
 __global__ void k_begin_step(InstParams* __restrict__ P, InstState* __restrict__ S, Globals* g,
                              uint32_t n_inst)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t t = g->t + 1;  // the step being taken (pot.c:4261)
-    if (k < n_inst) {
+    for (uint32_t k = threadIdx.x; k < n_inst; k += blockDim.x) {
         // first stepped index: the lowest index whose species is due
         // (pot.c:2430-2434); the grain itself is never stepped here
         uint32_t start = NONE;
@@ -91,7 +92,8 @@ __global__ void k_begin_step(InstParams* __restrict__ P, InstState* __restrict__
             S[k].acc_p_hi[c] = 0ull;
         }
     }
-    if (k == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
         g->t = t;
         g->tile_ctr = 0;
         g->tile_ctr2 = 0;
@@ -688,7 +690,7 @@ __global__ __launch_bounds__(1024) void k_level_moments_block(
 
 void launch_begin_step(Bufs& b, hipStream_t s)
 {
-    k_begin_step<<<grid(b.n_inst), BT, 0, s>>>(b.P, b.S, b.g, b.n_inst);
+    k_begin_step<<<1, BT, 0, s>>>(b.P, b.S, b.g, b.n_inst);
 }
 
 // the leaf records again, from the current state: between the two fields

@@ -1555,7 +1555,7 @@ __global__ __launch_bounds__(64) void k_chi_b(const InstParams* __restrict__ P, 
 {
     if ((g->t % 20) != 0 || step_failed(g)) return;
     const uint32_t inst = blockIdx.x;
-    if (inst >= n_inst || threadIdx.x != 0) return;
+    if (inst >= n_inst || threadIdx.x != 0 || S[inst].retired) return;
     const InstParams& p = P[inst];
     const double x = p.chi_pt[0], y = p.chi_pt[1], z = p.chi_pt[2];
     const double theta2 = p.theta2, soft2 = p.soft2;

@@ -32,7 +32,7 @@ EXPORTS = ["potg_create", "potg_destroy", "potg_last_error", "potg_upload", "pot
            "potg_phi_at", "potg_direct_sum", "potg_stats", "potg_stats_reset",
            "potg_stage_times", "potg_sync", "potg_tree_canon_dipoles", "potg_tune",
            "potg_set_tuning", "potg_set_counting", "potg_check_particles",
-           "potg_phi_at_noalloc"]
+           "potg_phi_at_noalloc", "potg_set_retire", "potg_instance_retired"]
 
 
 class Config(C.Structure):
@@ -110,6 +110,9 @@ def lib():
     L.potg_set_tuning.argtypes = [P, C.POINTER(Tuning), C.POINTER(Tuning)]
     L.potg_check_particles.argtypes = [C.POINTER(Config), P, C.POINTER(C.c_uint64)]
     L.potg_phi_at_noalloc.argtypes = [P, C.c_uint32, P, C.c_size_t, P, P]
+    L.potg_set_retire.argtypes = [P, C.c_int32]
+    L.potg_instance_retired.argtypes = [P, C.c_uint32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_uint64)]
     for n in EXPORTS:
         if n != "potg_last_error":
             getattr(L, n).restype = C.c_int
@@ -288,6 +291,16 @@ class Context:
 
     def set_counting(self, on):
         _check(self.L.potg_set_counting(self.h, 1 if on else 0))
+
+    def set_retire(self, on):
+        """A failed reinjection retires its instance instead of failing the step."""
+        _check(self.L.potg_set_retire(self.h, 1 if on else 0))
+
+    def retired(self, inst):
+        """(retired, step of the failure) of instance inst"""
+        r, t = C.c_int32(), C.c_uint64()
+        _check(self.L.potg_instance_retired(self.h, inst, C.byref(r), C.byref(t)))
+        return bool(r.value), int(t.value)
 
     def stage_times(self, enable=1):
         ms = np.zeros(5)

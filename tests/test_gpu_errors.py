@@ -12,11 +12,15 @@ behind on the device.
   potg_upload refuses it on the host; a NaN that arises on the device (a
   NaN velocity carries the position out at the next drift) is reported by
   the next step's tree build as POTG_ERR_OUT_OF_BOX.
+- A chi_b that bars a species from entering: the reference's SCEPTIC
+  reinjection recurses without end (pot.c:583-598); the GPU's returns
+  POTG_ERR_REINJECT after 1000 tries, or, with potg_set_retire, retires the
+  instance and lets the others go on.
 """
 import numpy as np
 import pytest
 
-from gpu_common import big_case, gpu_ctx, oracle_sim
+from gpu_common import big_case, domain_radius, gpu_ctx, oracle_sim
 
 from pot_amd import potg as G
 
@@ -107,3 +111,82 @@ def test_errors_name_the_instance_and_particle(precision):
     ctx.upload(1, parts)
     ctx.upload(0, parts)
     assert ctx.step()[0]["n_pushed"] > 0
+
+
+# chi_b at which no electron entry is admissible: with Te = Ti the electrons
+# see chi = +30, and the oracle's sampler (oracle/pot_oracle.c, the
+# reference's retry loop) runs through any supply of uniforms without an
+# admissible alpha; ions at chi_b = -30 enter after a few dozen draws
+CHI_BARS_ELECTRONS = -30.0
+BIG = big_case(20001, seed=72)
+
+
+def _barred_pair(precision, retire):
+    from gpu_common import gpu_config
+    parts = oracle_sim(BIG).particles()
+    cfg = gpu_config(BIG, seed=3)
+    ctx = G.Context([cfg, cfg], precision)
+    ctx.set_counting(False)
+    ctx.set_retire(retire)
+    ctx.upload(0, parts)
+    ctx.upload(1, parts, chi_b=CHI_BARS_ELECTRONS)
+    return ctx, parts, cfg
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_barred_reinjection_is_reported(precision):
+    ctx, parts, _ = _barred_pair(precision, False)
+    with pytest.raises(G.PotgError,
+                       match=r"POTG_ERR_REINJECT.*instance 1, particle \d+\) found no admissible"):
+        for _ in range(10):  # electrons leave the domain every step
+            ctx.step()
+    assert ctx.retired(1) == (False, 0)
+    # the failed particle kept its state outside the domain: new uploads of
+    # both instances step again
+    ctx.upload(1, parts)
+    _recovers(ctx, parts)
+
+
+@pytest.mark.parametrize("precision", [G.POTG_FP64, G.POTG_MIXED], ids=["fp64", "mixed"])
+def test_barred_reinjection_retires_its_instance(precision):
+    """With potg_set_retire the failing instance stops where it failed (its
+    particles frozen, the failing ones parked at rest inside the grain) and
+    the other instance goes on exactly as it would alone."""
+    ctx, parts, cfg = _barred_pair(precision, True)
+    alone = G.Context([cfg], precision)
+    alone.set_counting(False)
+    alone.upload(0, parts)
+    k = 0
+    while not ctx.retired(1)[0]:
+        k += 1
+        assert k <= 10, "no electron was reinjected in 10 steps"
+        ctx.step()
+        alone.step()
+    assert ctx.retired(1) == (True, k)
+    assert ctx.retired(0) == (False, 0)
+    frozen, g_frozen = ctx.download(1)
+    for _ in range(5):
+        ctx.step()
+        alone.step()
+    later, g_later = ctx.download(1)
+    assert np.array_equal(frozen.view(np.uint8), later.view(np.uint8))
+    assert g_later["q"] == g_frozen["q"] and g_later["chi_b"] == g_frozen["chi_b"]
+    # the parked particles: electrons at rest within half the grain radius of
+    # the centre, each at its own place
+    R = domain_radius(BIG)
+    a = float(BIG["args"]["-a"])
+    d = np.sqrt((later["x"] - R) ** 2 + (later["y"] - R) ** 2 + (later["z"] - R) ** 2)
+    parked = (d < 0.5 * a + 1e-12 * R) & (np.arange(len(later)) > 0)
+    assert parked.sum() >= 1
+    assert np.all(later["species"][parked] == G.ELECTRON)
+    assert np.all(later["vx"][parked] == 0) and np.all(later["vz"][parked] == 0)
+    assert len(np.unique(later["x"][parked])) == parked.sum()
+    # the other instance: bit for bit its run alone
+    pa, ga = ctx.download(0)
+    pb, gb = alone.download(0)
+    assert np.array_equal(pa.view(np.uint8), pb.view(np.uint8))
+    assert ga == gb
+    # a new upload brings the instance back
+    ctx.upload(1, parts)
+    assert ctx.retired(1) == (False, 0)
+    assert ctx.step()[1]["n_pushed"] > 0

@@ -384,6 +384,39 @@ def test_batched_instances_equal_single_runs():
         assert gb == gs, k
 
 
+def test_many_instances_keep_one_step_count():
+    """More instances than one wavefront holds (72): every instance's step
+    starts from the same step count, so the ion sub-cycle (tss 42 at
+    Te = Ti) falls on the same steps in each.  Two instances, the first and
+    the last, equal their single runs over 43 steps, across the steps (42,
+    43) where ions are and are not due."""
+    from gpu_common import gpu_config
+    case = big_case(601, seed=81)
+    cfgs = [gpu_config(case, seed=90 + k) for k in range(72)]
+    state = oracle_sim(case).particles()
+    batch = G.Context(cfgs, G.POTG_FP64)
+    batch.set_counting(False)
+    for k in range(72):
+        batch.upload(k, state)
+    singles = {}
+    for k in (0, 71):
+        singles[k] = G.Context([cfgs[k]], G.POTG_FP64)
+        singles[k].set_counting(False)
+        singles[k].upload(0, state)
+    pushed = []
+    for _ in range(43):
+        out = batch.step()
+        pushed.append(out[71]["n_pushed"])
+        for s in singles.values():
+            s.step()
+    assert pushed[41] > pushed[40] and pushed[42] < pushed[41]  # ions due at step 42
+    for k, s in singles.items():
+        pb, gb = batch.download(k)
+        ps, gs = s.download(0)
+        assert np.array_equal(pb.view(np.uint8), ps.view(np.uint8)), k
+        assert gb == gs, k
+
+
 def test_upload_order_and_reupload_keep_instances_apart():
     """Instances uploaded in any order, one of them re-uploaded after
     stepping: each instance's particles stay its own (the upload returns the
