@@ -387,34 +387,44 @@ def test_batched_instances_equal_single_runs():
 def test_many_instances_keep_one_step_count():
     """More instances than one wavefront holds (72): every instance's step
     starts from the same step count, so the ion sub-cycle (tss 42 at
-    Te = Ti) falls on the same steps in each.  Two instances, the first and
-    the last, equal their single runs over 43 steps, across the steps (42,
-    43) where ions are and are not due."""
+    Te = Ti) falls on the same steps in each, and every instance equals its
+    single run over 43 steps, across the steps (42, 43) where ions are and
+    are not due.  With so few particles (601) chi_b, measured at steps 20
+    and 40, can bar a species (the reference's pathology, pot.c:583-598):
+    batch and singles retire such an instance (potg_set_retire), at the
+    same step."""
     from gpu_common import gpu_config
     case = big_case(601, seed=81)
     cfgs = [gpu_config(case, seed=90 + k) for k in range(72)]
     state = oracle_sim(case).particles()
     batch = G.Context(cfgs, G.POTG_FP64)
     batch.set_counting(False)
+    batch.set_retire(True)
     for k in range(72):
         batch.upload(k, state)
-    singles = {}
-    for k in (0, 71):
-        singles[k] = G.Context([cfgs[k]], G.POTG_FP64)
-        singles[k].set_counting(False)
-        singles[k].upload(0, state)
+    singles = []
+    for k in range(72):
+        s = G.Context([cfgs[k]], G.POTG_FP64)
+        s.set_counting(False)
+        s.set_retire(True)
+        s.upload(0, state)
+        singles.append(s)
     pushed = []
     for _ in range(43):
-        out = batch.step()
-        pushed.append(out[71]["n_pushed"])
-        for s in singles.values():
+        pushed.append([o["n_pushed"] for o in batch.step()])
+        for s in singles:
             s.step()
-    assert pushed[41] > pushed[40] and pushed[42] < pushed[41]  # ions due at step 42
-    for k, s in singles.items():
+    live = [k for k in range(72) if not batch.retired(k)[0]]
+    assert len(live) >= 36, live
+    k = live[-1]
+    assert pushed[41][k] > pushed[40][k] and pushed[42][k] < pushed[41][k]  # ions due at 42
+    for k, s in enumerate(singles):
+        assert batch.retired(k) == s.retired(0), k
         pb, gb = batch.download(k)
         ps, gs = s.download(0)
         assert np.array_equal(pb.view(np.uint8), ps.view(np.uint8)), k
         assert gb == gs, k
+        s.close()
 
 
 def test_upload_order_and_reupload_keep_instances_apart():
