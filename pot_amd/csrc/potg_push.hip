@@ -741,7 +741,9 @@ __global__ void k_finish(const InstParams* __restrict__ P, InstState* S, Globals
         s.out_i = s.acc_i;
         // every index from the first stepped one to the last was pushed
         // (pot.c:2436-2440)
-        s.out_pushed = s.start == NONE ? 0u : p.N - s.start;
+        // (none once the instance retired in an earlier step)
+        const bool frozen = s.retired && s.retired_t < g->t;
+        s.out_pushed = (s.start == NONE || frozen) ? 0u : p.N - s.start;
         s.out_reinj_abs = s.n_reinj_abs;
         s.out_reinj_out = s.n_reinj_out;
         s.out_wall_inward = s.n_wall_inward;

@@ -166,8 +166,9 @@ def test_barred_reinjection_retires_its_instance(precision):
     assert ctx.retired(0) == (False, 0)
     frozen, g_frozen = ctx.download(1)
     for _ in range(5):
-        ctx.step()
+        out = ctx.step()
         alone.step()
+        assert out[1]["n_pushed"] == 0 and out[0]["n_pushed"] > 0
     later, g_later = ctx.download(1)
     assert np.array_equal(frozen.view(np.uint8), later.view(np.uint8))
     assert g_later["q"] == g_frozen["q"] and g_later["chi_b"] == g_frozen["chi_b"]
