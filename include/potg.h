@@ -272,7 +272,7 @@ int potg_instance_retired(potg_ctx* ctx, uint32_t inst, int32_t* retired, uint64
 
 /* Interaction counters (n_cell, n_leaf, n_open, n_range_loads) of the step
    walks on (on != 0, the default) or off.  They are instrumentation: off,
-   the float32 walk runs 10-16% faster; point queries count either way.
+   the float32 walk runs about 8-10% faster; point queries count either way.
    No counterpart in the reference. */
 int potg_set_counting(potg_ctx* ctx, int32_t on);
 
