@@ -69,6 +69,9 @@ build_sph() {  # name shape theta bx by bz [extra -D...]; NO_BORIS=1: velocity V
 
 # spherical grain, SCEPTIC reinjection, Boris
 build_pot pot_sphere 0.5 0.0 0.0 0.0
+# the same seeding from /dev/urandom, as shipped, for runs on several ranks (one
+# seed file would give every rank the same stream, so coincident particles)
+SEED_SOURCE='"/dev/urandom"' build_pot pot_sphere_mpi 0.5 0.0 0.0 0.0
 build_pot pot_sphere_bz 0.5 0.0 0.0 50.0
 build_pot pot_sphere_bx 0.5 30.0 0.0 0.0
 build_pot pot_sphere_t08 0.8 0.0 0.0 0.0
