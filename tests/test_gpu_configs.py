@@ -233,6 +233,40 @@ def test_sweep_captured_steps_conserve_per_instance(sweep):
 
 # ---- the 4M prolate workload ---------------------------------------------------
 
+def test_prolate_4m_headline_walk_matches_oracle():
+    """The headline's own setting: the 4M workload with DIPOLES at theta 0.14.
+    The tree with its dipole moments bit for bit; the float64 walk equals
+    the oracle's (exact interaction counts, <= 1e-13); the float32 walk the
+    4M context runs untuned (whole-range sweeps) within the mixed walk's
+    tolerance (median 1e-5, 99th percentile 2e-3: MAC decisions that flip at
+    the threshold in float32)."""
+    case = dict(_case(W.prolate_4m_tilted_b(), "prolate_4m_tilted_b", 6), theta=0.14, dipoles=1)
+    sim = oracle_sim(case)
+    parts = sim.particles()
+    sim.build_tree()
+    d64 = gpu_ctx(case, parts)
+    d64.build_tree()
+    ref = sim.tree_canon()
+    _canon_equal(d64.tree_canon(0), ref)
+    assert np.array_equal(d64.tree_dipoles(0), sim.tree_dipoles())
+    del ref
+    idx = _sample(len(parts), 384, 2)
+    xyz, Eo, cnto = _oracle_walk(sim, parts, idx)
+    E, cnt = d64.field_at(0, xyz, exclude=idx, counts=True)
+    assert np.array_equal(cnt.astype(np.int64), cnto), "interaction counts differ"
+    rel = np.linalg.norm(E - Eo, axis=1) / np.linalg.norm(Eo, axis=1)
+    assert rel.max() <= 1e-13, rel.max()
+    d64.close()
+    mixed = gpu_ctx(case, parts, precision=G.POTG_MIXED)
+    t = mixed.tuning()
+    assert (t["group"], t["variant"]) == (64, 1)
+    mixed.build_tree()
+    E32 = mixed.field_at(0, xyz, exclude=idx)
+    rel32 = np.linalg.norm(E32 - Eo, axis=1) / np.linalg.norm(Eo, axis=1)
+    assert np.median(rel32) < 1e-5, np.median(rel32)
+    assert np.percentile(rel32, 99) < 2e-3, np.percentile(rel32, 99)
+
+
 def test_prolate_4m_tree_and_walk_match_oracle():
     case = _case(W.prolate_4m_tilted_b(), "prolate_4m_tilted_b", 6)
     sim = oracle_sim(case)
