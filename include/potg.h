@@ -147,10 +147,13 @@ typedef struct potg_stats_t {
 } potg_stats_t;
 
 /* Launch parameters of the field walk (potg_tune picks them by timing).
-   Until it or potg_set_tuning runs they follow the context's size: from
-   about 0.5M particles 64 targets per wavefront (sweeping whole ranges in
-   the float32 walk) and 8 blocks per CU, below that 16 targets per
-   wavefront (64 for the float64 walk) and 4 blocks per CU. */
+   Until it or potg_set_tuning runs they follow the context's size, in
+   64-target tiles per CU (half the particles are targets): the float32
+   walk sweeps whole ranges with 64 targets per wavefront on 8 blocks per CU
+   from 16 tiles per CU (about 0.5M particles), on 2 blocks per CU from 1
+   tile per CU, and below that takes 8 targets per wavefront on 2 blocks
+   per CU; the float64 walk takes 64 targets on 8 blocks per CU from 16
+   tiles per CU, 4 below. */
 typedef struct potg_tuning {
     uint32_t walk_blocks; /* blocks (4 wavefronts each) of the persistent walk */
     uint32_t group;       /* float32 walk: targets per wavefront, 8, 16 or 64;

@@ -354,18 +354,30 @@ int potg_create(const potg_config* configs, uint32_t n_inst, int32_t precision, 
     while ((1u << ib) < n_inst) ib++;
     b.key_levels = (63 - ib) / 3 < 21 ? (63 - ib) / 3 : 21;
     b.key_bits = 64;
-    // defaults until potg_tune or potg_set_tuning, by size: with enough
-    // 64-target tiles to give every SIMD several wavefronts (about 0.5M
-    // particles and up, half of them stepped each step), 64 targets per
-    // wavefront sweeping whole ranges and 8 blocks of 4 waves per CU; below
-    // that the tiles would leave most of the chip idle, so 16 targets per
-    // wavefront and 4 blocks per CU
+    // defaults until potg_tune or potg_set_tuning, by the number of 64-target
+    // tiles per CU (half the particles are stepped in a step):
+    // - 16 or more (about 0.5M particles and up): 64 targets per wavefront
+    //   sweeping whole ranges, 8 blocks of 4 waves per CU;
+    // - 1 to 16: the same walk on 2 blocks per CU, fewer wavefronts each
+    //   with more tiles to take;
+    // - below 1 (under about 32k particles) the tiles would leave most SIMDs
+    //   idle: 8 targets per wavefront with 8 children evaluated at once, 2
+    //   blocks per CU.
+    // The float64 walk has one shape: 64 targets, 8 blocks per CU from 16
+    // tiles per CU, 4 below.
     {
+        const uint64_t cus = (uint64_t)device_cus();
         const uint64_t tiles = n_total / 128u;
-        const bool large = tiles >= 16u * (uint64_t)device_cus();
-        b.walk_blocks = (large ? 8u : 4u) * (uint32_t)device_cus();
-        b.group = (large || precision == POTG_FP64) ? 64u : 16u;  // float64: always 64
-        b.variant = (large && precision == POTG_MIXED) ? 1u : 0u;
+        const bool large = tiles >= 16u * cus, tiny = tiles < cus;
+        if (precision == POTG_MIXED) {
+            b.walk_blocks = (large ? 8u : 2u) * (uint32_t)cus;
+            b.group = tiny ? 8u : 64u;
+            b.variant = tiny ? 0u : 1u;
+        } else {
+            b.walk_blocks = (large ? 8u : 4u) * (uint32_t)cus;
+            b.group = 64u;
+            b.variant = 0u;
+        }
     }
 
     c->hp.resize(n_inst);
@@ -1007,7 +1019,7 @@ int potg_tune(potg_ctx* c, potg_tuning* out)
     // candidates: targets per wavefront and visit order (float32 walk only:
     // 8, 16, or 64 in either order) x blocks per CU
     const uint32_t shapes[][2] = {{8, 0}, {16, 0}, {64, 0}, {64, 1}};
-    const uint32_t bpc[] = {4, 8, 16};
+    const uint32_t bpc[] = {2, 4, 8, 16};
     const uint32_t keep_wb = b.walk_blocks, keep_gr = b.group, keep_var = b.variant;
     double best = 1e30, keep_ms = -1.0;
     uint32_t best_wb = keep_wb, best_gr = keep_gr, best_var = keep_var;

@@ -155,6 +155,9 @@ def test_barred_reinjection_retires_its_instance(precision):
     ctx, parts, cfg = _barred_pair(precision, True)
     alone = G.Context([cfg], precision)
     alone.set_counting(False)
+    # the batch's launch shape (the untuned one follows the context's size)
+    t = ctx.tuning()
+    alone.set_tuning(t["walk_blocks"], t["group"], t["variant"])
     alone.upload(0, parts)
     k = 0
     while not ctx.retired(1)[0]:

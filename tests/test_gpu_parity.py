@@ -210,7 +210,7 @@ def test_tuning_takes_no_step_and_is_reproducible():
         after = a.stats()
         for k in ("steps", "n_cell", "n_leaf", "n_open", "n_range_loads"):
             assert after[k] == before[k], k
-        b.set_tuning(t["walk_blocks"], t["group"])
+        b.set_tuning(t["walk_blocks"], t["group"], t["variant"], t["sort_tile"])
         for _ in range(3):
             a.step()
             b.step()

@@ -80,22 +80,25 @@ def test_sweep_steps_conserve_and_count(dipoles):
 
 def test_untuned_shape_follows_size():
     """Until potg_tune or potg_set_tuning the walk's shape follows the
-    context's size: from 16 64-target tiles per CU (about 0.5M particles
-    with the 4-block-per-CU rule) the float32 walk sweeps whole ranges with
-    64 targets per wavefront on 8 blocks per CU; below, 16 targets per
-    wavefront on 4 blocks per CU; the float64 walk always takes 64."""
-    small = gpu_config(big_case(20001))
-    big = gpu_config(big_case(1500001))
+    context's 64-target tiles per CU (half the particles are targets): from
+    16 per CU the float32 walk sweeps whole ranges with 64 targets per
+    wavefront on 8 blocks per CU, from 1 per CU the same walk on 2 blocks
+    per CU, below that 8 targets per wavefront on 2 blocks per CU; the
+    float64 walk takes 64 targets on 8 blocks per CU from 16 tiles per CU,
+    4 below."""
     shapes = {}
-    for name, cfg in (("small", small), ("big", big)):
+    for name, n in (("tiny", 20001), ("middle", 200001), ("large", 1500001)):
+        cfg = gpu_config(big_case(n))
         for prec in (G.POTG_MIXED, G.POTG_FP64):
             ctx = G.Context([cfg], prec)
             shapes[name, prec] = ctx.tuning()
             ctx.close()
-    b_mixed, b_fp64 = shapes["big", G.POTG_MIXED], shapes["big", G.POTG_FP64]
-    s_mixed, s_fp64 = shapes["small", G.POTG_MIXED], shapes["small", G.POTG_FP64]
-    assert (b_mixed["group"], b_mixed["variant"]) == (64, 1)
-    assert b_fp64["group"] == 64 and b_fp64["walk_blocks"] == b_mixed["walk_blocks"]
-    assert (s_mixed["group"], s_mixed["variant"]) == (16, 0)
-    assert s_fp64["group"] == 64
-    assert b_mixed["walk_blocks"] == 2 * s_mixed["walk_blocks"]
+    cus = shapes["large", G.POTG_MIXED]["walk_blocks"] // 8
+    assert cus * 8 == shapes["large", G.POTG_MIXED]["walk_blocks"]
+    # 200001 particles: at least one tile per CU on a device of up to 781 CUs
+    want = {("large", G.POTG_MIXED): (64, 1, 8), ("middle", G.POTG_MIXED): (64, 1, 2),
+            ("tiny", G.POTG_MIXED): (8, 0, 2), ("large", G.POTG_FP64): (64, 0, 8),
+            ("middle", G.POTG_FP64): (64, 0, 4), ("tiny", G.POTG_FP64): (64, 0, 4)}
+    for key, (group, variant, bpc) in want.items():
+        t = shapes[key]
+        assert (t["group"], t["variant"], t["walk_blocks"]) == (group, variant, bpc * cus), (key, t)
